@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Trotter amplitude-updates/s and % of HBM peak on one MI355X.
+
+Workload (config "large plain run"): N=30 statevector in complex64, trap drawn
+from a seed (mt19937_64 -> steepest descent), second-order (Strang) steps with
+dt=0.02 and g=0.35, P_< and <d_H> reduced in FP64 and recorded every step.
+A step = one Trotter step of the whole 2^30-amplitude state (phase, 30 qubit
+rotations, fused reductions).  value = 2^N * steps / seconds ("amp-updates/s":
+one amplitude through one full Trotter step).
+
+Prints ONE JSON line.  The roofline object is measured live with HIP events
+around every sweep launch on the library's stream; the cpu_baseline object
+times the FP64 CPU restatement (oracle) on a bounded sample of the same
+workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+
+
+def _cpu_baseline(n: int, seed: int, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # CPU restatement, used only as the timed baseline here
+    start, s0, e0 = O.make_trap(n, seed)
+    sec = O.trotter_bench(n, s0, 0.35, 0.02, 1, order=2, threads=threads)
+    return {
+        "value": float((1 << n) / sec),
+        "unit": "amp-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"1 Strang step of the N={n} trap (same workload), FP64 complex128 "
+                  f"statevector restatement of sampler.cuh (int energy cache, per-qubit "
+                  f"passes, OpenMP {threads} threads); setup excluded; {sec:.2f} s",
+    }
+
+
+def _traffic():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic_n30_c64.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("traffic_bytes_per_launch")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--precision", choices=["c64", "c128"], default="c64")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--g", type=float, default=0.35)
+    ap.add_argument("--dt", type=float, default=0.02)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=30)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): the "
+              f"statevector path has no multi-GPU sharding in this build.", file=sys.stderr)
+        return 2
+
+    bk = importlib.import_module("bernasconi-krylov_amd")
+    prec = bk.BK_C64 if args.precision == "c64" else bk.BK_C128
+    n = args.n
+    start, s0, e0 = bk.make_trap(n, args.seed)
+    eng = bk.Engine(0)
+
+    # warmup (fresh start from the trap), then the timed continuation
+    print(f"[bench] N={n} {args.precision} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
+    eng.evolve(n, s0, args.g, args.dt, args.warmup, order=bk.BK_STRANG, precision=prec,
+               init=True, measure_every=1)
+    import torch  # device plumbing only: synchronisation around the timed region
+    torch.cuda.synchronize()
+    eng.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = eng.evolve(n, s0, args.g, args.dt, args.steps, order=bk.BK_STRANG, precision=prec,
+                     init=False, measure_every=1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rows = eng.timing_read()
+    eng.timing_enable(False)
+    sec = t1 - t0
+    ms_per_step = 1e3 * sec / args.steps
+    value = float((1 << n) * args.steps / sec)
+
+    dom = max(rows, key=lambda r: r["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    achieved = dom["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+    step_bytes = sum(r["bytes_per_launch"] * r["launches"] for r in rows) / args.steps
+    step_dev_ms = sum(r["total_ms"] for r in rows) / args.steps
+    kernels = {r["name"]: {"avg_ms": round(r["total_ms"] / r["launches"], 4),
+                           "launches": r["launches"],
+                           "gbs": round(r["bytes_per_launch"] / (r["total_ms"] / r["launches"])
+                                        * 1e-6, 1)} for r in rows}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline N={args.cpu_n} threads={threads}", file=sys.stderr,
+              flush=True)
+        cpu = _cpu_baseline(args.cpu_n, args.seed, threads)
+
+    out = {
+        "metric": "Trotter amp-updates/s & % HBM peak, N=30 c64; adjoint/forward time ratio",
+        "value": value,
+        "unit": "amp-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (seed-drawn steepest-descent trap, no external data)",
+        "config": {
+            "workload": f"N={n} {args.precision} Strang dt={args.dt} g={args.g} trotter step, "
+                        f"P_< and <d_H> (FP64) fused every step",
+            "n": n, "trap_seed": args.seed, "trap_bits": hex(s0), "E0": e0,
+            "p_less_final": float(res["p_less"][-1]), "norm_final": float(res["norm"][-1]),
+            "step_hbm_bytes": step_bytes,
+            "step_hbm_frac": step_bytes / (step_dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "kernels": kernels,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom["name"],
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": _traffic(),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,973 @@
+// bk_capi.cpp -- C ABI of libbk.so (declared in include/bk_capi.h).
+//
+// Host orchestration in C++17 over the gfx950 kernels: a context owns one
+// device, one HIP stream (default priority), the statevector and the scratch
+// buffers.  The Trotter loop enqueues kernels only; observables land in a
+// device-side series buffer that is copied back once per call, so there is
+// no host synchronisation per step (the reference syncs and copies two
+// doubles per measurement, sampler.cuh:369-380).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <random>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/bk_capi.h"
+#include "bk_host.hpp"
+#include "bk_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCALL(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return fail(_e == hipErrorOutOfMemory ? BK_E_NOMEM : BK_E_HIP, "%s failed: %s (%s:%d)", \
+                        #expr, hipGetErrorString(_e), __FILE__, __LINE__);                      \
+    } while (0)
+
+size_t amp_bytes(int precision) { return precision == BK_C64 ? 8 : 16; }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+// HIP-event timer: events are drawn from a pool, recorded around each launch,
+// and resolved into per-name totals when read.
+struct EventTimer : bk::KernelTimer {
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    struct Rec {
+        size_t ev0, ev1;
+        std::string name;
+        double bytes;
+    };
+    std::vector<Rec> recs;
+    size_t open_ev = 0;
+    hipError_t err = hipSuccess;
+    hipEvent_t take()
+    {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) {
+                err = r;
+                return nullptr;
+            }
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(hipStream_t s) override
+    {
+        open_ev = used;
+        hipEvent_t e = take();
+        if (e) (void)hipEventRecord(e, s);
+    }
+    void end(hipStream_t s, const char* name, double bytes) override
+    {
+        const size_t i1 = used;
+        hipEvent_t e = take();
+        if (e) (void)hipEventRecord(e, s);
+        if (err == hipSuccess) recs.push_back({open_ev, i1, name, bytes});
+    }
+    void reset()
+    {
+        used = 0;
+        recs.clear();
+        err = hipSuccess;
+    }
+    ~EventTimer() override
+    {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+struct bk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf psi;
+    int n = 0;
+    int precision = 0;
+    bool has_state = false;
+    double pending_dtp = 0.0;  // closing Strang half-phase owed by the held state
+    DevBuf partials;           // 3 doubles per workgroup of the sweep
+    DevBuf series;             // 3 doubles per measurement slot
+    DevBuf count_partials;
+    DevBuf count_out;
+    DevBuf samp_sums, samp_prefix, samp_rand, samp_out;
+    DevBuf blas_partials, blas_out;
+    std::map<std::pair<int, std::pair<int, double>>, DevBuf> phase_tabs;  // (n, (prec, dtp))
+    bk::SweepTuning tuning{};
+    bool timing = false;
+    EventTimer timer;
+};
+
+namespace {
+
+int check_ctx(bk_ctx* c)
+{
+    if (!c) return fail(BK_E_INVALID, "null context");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return fail(BK_E_HIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e));
+    return BK_OK;
+}
+
+// exp(-i dtp E / n) for E = 0..Emax in the state's precision.  The argument is
+// -(dtp/n)*E as in sampler.cuh:83, evaluated and reduced mod 2*pi in FP64 by
+// the C library's cos/sin, then rounded once to the table precision.
+int get_phase_tab(bk_ctx* c, int n, int precision, double dtp, const void** out)
+{
+    auto key = std::make_pair(n, std::make_pair(precision, dtp));
+    auto it = c->phase_tabs.find(key);
+    if (it != c->phase_tabs.end()) {
+        *out = it->second.p;
+        return BK_OK;
+    }
+    const int64_t emax = static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6;  // sum_{j<n} j^2
+    const size_t cnt = static_cast<size_t>(emax) + 1;
+    const double w = -(dtp / static_cast<double>(n));
+    DevBuf buf;
+    if (precision == BK_C64) {
+        std::vector<float> h(2 * cnt);
+        for (size_t e = 0; e < cnt; ++e) {
+            const double th = w * static_cast<double>(e);
+            h[2 * e] = static_cast<float>(std::cos(th));
+            h[2 * e + 1] = static_cast<float>(std::sin(th));
+        }
+        HIPCALL(buf.ensure(h.size() * sizeof(float)));
+        HIPCALL(hipMemcpy(buf.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> h(2 * cnt);
+        for (size_t e = 0; e < cnt; ++e) {
+            const double th = w * static_cast<double>(e);
+            h[2 * e] = std::cos(th);
+            h[2 * e + 1] = std::sin(th);
+        }
+        HIPCALL(buf.ensure(h.size() * sizeof(double)));
+        HIPCALL(hipMemcpy(buf.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (c->phase_tabs.size() > 64) {  // bound the cache
+        for (auto& kv : c->phase_tabs) kv.second.release();
+        c->phase_tabs.clear();
+    }
+    *out = buf.p;
+    c->phase_tabs.emplace(key, buf);
+    return BK_OK;
+}
+
+int ensure_state(bk_ctx* c, int n, int precision)
+{
+    if (n < 1 || n > BK_MAX_N) return fail(BK_E_INVALID, "n=%d outside [1, %d]", n, BK_MAX_N);
+    if (precision != BK_C64 && precision != BK_C128) return fail(BK_E_INVALID, "precision=%d", precision);
+    const size_t bytes = amp_bytes(precision) << n;
+    if (c->psi.cap < bytes) {
+        c->psi.release();
+        HIPCALL(c->psi.ensure(bytes));
+    }
+    c->n = n;
+    c->precision = precision;
+    c->has_state = true;
+    c->pending_dtp = 0.0;
+    c->tuning = bk::default_tuning(n, precision);
+    c->tuning.timer = c->timing ? &c->timer : nullptr;
+    HIPCALL(c->partials.ensure(bk::partials_bytes(c->tuning.grid)));
+    return BK_OK;
+}
+
+// apply any owed closing half-phase so the held state is the exact one
+int flush_pending(bk_ctx* c)
+{
+    if (!c->has_state || c->pending_dtp == 0.0) return BK_OK;
+    const void* tab = nullptr;
+    int rc = get_phase_tab(c, c->n, c->precision, c->pending_dtp, &tab);
+    if (rc != BK_OK) return rc;
+    HIPCALL(bk::launch_phase(c->psi.p, c->n, c->precision, tab, c->stream));
+    c->pending_dtp = 0.0;
+    return BK_OK;
+}
+
+bool is_fin(double x) { return std::isfinite(x); }
+
+}  // namespace
+
+extern "C" {
+
+const char* bk_last_error(void) { return g_err.c_str(); }
+
+int bk_version(void) { return 100; }
+
+int bk_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int bk_ctx_create(int device, bk_ctx** out)
+{
+    if (!out) return fail(BK_E_INVALID, "null out pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(BK_E_NODEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return fail(BK_E_INVALID, "device %d of %d", device, count);
+    HIPCALL(hipSetDevice(device));
+    bk_ctx* c = new bk_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(BK_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return BK_OK;
+}
+
+void bk_ctx_destroy(bk_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->psi.release();
+    c->partials.release();
+    c->series.release();
+    c->count_partials.release();
+    c->count_out.release();
+    c->samp_sums.release();
+    c->samp_prefix.release();
+    c->samp_rand.release();
+    c->samp_out.release();
+    c->blas_partials.release();
+    c->blas_out.release();
+    for (auto& kv : c->phase_tabs) kv.second.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int64_t bk_energy(int n, uint64_t s) { return bk::host_energy(n, s); }
+
+int bk_d_min(int n, uint64_t s0, int64_t target, int max_d) { return bk::host_d_min(n, s0, target, max_d); }
+
+int64_t bk_known_optimal_energy(int n) { return bk::host_known_optimal(n); }
+
+int bk_make_trap(int n, uint64_t seed, uint64_t* start, uint64_t* s0, int64_t* e0)
+{
+    if (n < 1 || n > 64 || !start || !s0 || !e0) return fail(BK_E_INVALID, "bk_make_trap: bad argument");
+    bk::host_make_trap(n, seed, start, s0, e0);
+    return BK_OK;
+}
+
+int bk_state_init(bk_ctx* c, int n, int precision, uint64_t start_bits)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = ensure_state(c, n, precision))) return rc;
+    const uint64_t s = start_bits & ((1ull << n) - 1ull);
+    HIPCALL(bk::launch_init_basis(c->psi.p, n, precision, s, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    return BK_OK;
+}
+
+int bk_state_upload(bk_ctx* c, int n, int precision, const void* host)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!host) return fail(BK_E_INVALID, "null host buffer");
+    if ((rc = ensure_state(c, n, precision))) return rc;
+    HIPCALL(hipMemcpyAsync(c->psi.p, host, amp_bytes(precision) << n, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    return BK_OK;
+}
+
+int bk_state_download(bk_ctx* c, void* host)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!host) return fail(BK_E_INVALID, "null host buffer");
+    if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
+    if ((rc = flush_pending(c))) return rc;
+    HIPCALL(hipMemcpyAsync(host, c->psi.p, amp_bytes(c->precision) << c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    return BK_OK;
+}
+
+int bk_state_device(bk_ctx* c, void** d, void** st, int* n, int* precision)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
+    if ((rc = flush_pending(c))) return rc;
+    if (d) *d = c->psi.p;
+    if (st) *st = c->stream;
+    if (n) *n = c->n;
+    if (precision) *precision = c->precision;
+    return BK_OK;
+}
+
+int bk_count_below(bk_ctx* c, int n, int64_t e0, uint64_t* count)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 1 || n > BK_MAX_N || !count) return fail(BK_E_INVALID, "bk_count_below: bad argument");
+    if (e0 <= 0) {
+        *count = 0;
+        return BK_OK;
+    }
+    const uint32_t e = static_cast<uint32_t>(std::min<int64_t>(e0, 0xFFFFFFFFll));
+    HIPCALL(c->count_partials.ensure(sizeof(unsigned long long) * 4096));
+    HIPCALL(c->count_out.ensure(sizeof(unsigned long long)));
+    HIPCALL(bk::launch_count_below(n, e, c->count_out.as<unsigned long long>(),
+                                   c->count_partials.as<unsigned long long>(), 4096, c->stream));
+    unsigned long long h = 0;
+    HIPCALL(hipMemcpyAsync(&h, c->count_out.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    *count = h;
+    return BK_OK;
+}
+
+int bk_observables(bk_ctx* c, uint64_t s0, int64_t e0, double* out3)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!out3) return fail(BK_E_INVALID, "null output");
+    if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
+    HIPCALL(c->series.ensure(3 * sizeof(double)));
+    const uint32_t e = static_cast<uint32_t>(std::max<int64_t>(0, std::min<int64_t>(e0, 0xFFFFFFFFll)));
+    HIPCALL(bk::launch_measure(c->psi.p, c->n, c->precision, static_cast<uint32_t>(s0), e, c->series.as<double>(),
+                               c->partials.as<double>(), c->tuning.grid, c->stream));
+    HIPCALL(hipMemcpyAsync(out3, c->series.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    return BK_OK;
+}
+
+// Enqueue one Trotter step of the held state.  measure_slot: device pointer or null.
+static int enqueue_step(bk_ctx* c, int order, double dt, double g, uint64_t s0, uint32_t e0, double* slot)
+{
+    double dtp;
+    if (order == BK_STRANG) {
+        dtp = c->pending_dtp + 0.5 * dt;  // merge the owed closing half with this opening half
+    } else {
+        if (c->pending_dtp != 0.0) {
+            int rc = flush_pending(c);
+            if (rc) return rc;
+        }
+        dtp = dt;
+    }
+    const void* tab = nullptr;
+    int rc = get_phase_tab(c, c->n, c->precision, dtp, &tab);
+    if (rc) return rc;
+    bk::StepSpec sp{};
+    sp.n = c->n;
+    sp.precision = c->precision;
+    sp.gdt = g * dt;
+    sp.dtp = dtp;
+    sp.phase_tab = tab;
+    sp.series_slot = slot;
+    sp.s0 = static_cast<uint32_t>(s0);
+    sp.e0 = e0;
+    HIPCALL(bk::launch_step(c->psi.p, sp, c->tuning, c->partials.as<double>(), c->stream));
+    c->pending_dtp = (order == BK_STRANG) ? 0.5 * dt : 0.0;
+    return BK_OK;
+}
+
+int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!p) return fail(BK_E_INVALID, "null params");
+    if (p->n < 1 || p->n > BK_MAX_N) return fail(BK_E_INVALID, "n=%d outside [1, %d]", p->n, BK_MAX_N);
+    if (p->order != BK_LIE && p->order != BK_STRANG) return fail(BK_E_INVALID, "order=%d", p->order);
+    if (!is_fin(p->dt) || !is_fin(p->g) || p->steps < 0) return fail(BK_E_INVALID, "non-finite dt/g or steps < 0");
+    if (p->g_steps)
+        for (int k = 0; k < p->steps; ++k)
+            if (!is_fin(p->g_steps[k])) return fail(BK_E_INVALID, "non-finite g_steps[%d]", k);
+    const uint64_t mask = (1ull << p->n) - 1ull;
+    const uint64_t s0 = p->start_bits & mask;
+    const int64_t e0 = p->e0 >= 0 ? p->e0 : bk::host_energy(p->n, s0);
+    if (p->init_from_start) {
+        if ((rc = bk_state_init(c, p->n, p->precision, s0))) return rc;
+    } else {
+        if (!c->has_state) return fail(BK_E_NOSTATE, "init_from_start=0 but no state held");
+        if (c->n != p->n || c->precision != p->precision)
+            return fail(BK_E_INVALID, "held state is n=%d prec=%d", c->n, c->precision);
+    }
+    // measurement slots
+    const int every = p->measure_every;
+    int64_t slots = 1;
+    if (every > 0) slots += (p->steps + every - 1) / every;
+    if (out && out->capacity < slots && (out->p_less || out->dh || out->norm))
+        return fail(BK_E_INVALID, "series capacity %lld < %lld", static_cast<long long>(out->capacity),
+                    static_cast<long long>(slots));
+    HIPCALL(c->series.ensure(3 * sizeof(double) * static_cast<size_t>(slots)));
+    double* ds = c->series.as<double>();
+    const uint32_t e0u = static_cast<uint32_t>(std::max<int64_t>(0, std::min<int64_t>(e0, 0xFFFFFFFFll)));
+    int64_t slot = 0;
+    for (int k = 0; k < p->steps; ++k) {
+        double* sl = nullptr;
+        if (every > 0 && k % every == 0) sl = ds + 3 * (slot++);
+        const double g = p->g_steps ? p->g_steps[k] : p->g;
+        if ((rc = enqueue_step(c, p->order, p->dt, g, s0, e0u, sl))) return rc;
+    }
+    HIPCALL(bk::launch_measure(c->psi.p, c->n, c->precision, static_cast<uint32_t>(s0), e0u, ds + 3 * (slot++),
+                               c->partials.as<double>(), c->tuning.grid, c->stream));
+    if (out) {
+        std::vector<double> h(3 * static_cast<size_t>(slot));
+        HIPCALL(hipMemcpyAsync(h.data(), ds, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < slot; ++i) {
+            if (out->p_less) out->p_less[i] = h[3 * i];
+            if (out->dh) out->dh[i] = h[3 * i + 1];
+            if (out->norm) out->norm[i] = h[3 * i + 2];
+        }
+        out->count = slot;
+        out->e0 = e0;
+        out->p_blind = -1.0;  // only computed for a fresh start (a continuation keeps E0)
+        if (p->init_from_start) {
+            uint64_t cnt = 0;
+            if ((rc = bk_count_below(c, p->n, e0, &cnt))) return rc;
+            out->p_blind = static_cast<double>(cnt) / static_cast<double>(1ull << p->n);
+        }
+    } else {
+        HIPCALL(hipStreamSynchronize(c->stream));
+    }
+    return BK_OK;
+}
+
+int bk_sample(bk_ctx* c, const double* randnums, int64_t shots, int64_t* out_idx)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (shots < 0 || (shots > 0 && (!randnums || !out_idx))) return fail(BK_E_INVALID, "bk_sample: bad argument");
+    if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
+    if (shots == 0) return BK_OK;
+    if (shots > (1ll << 30)) return fail(BK_E_INVALID, "too many shots");
+    const size_t nsc = bk::sample_scratch_doubles(c->n);
+    HIPCALL(c->samp_sums.ensure(nsc * sizeof(double)));
+    HIPCALL(c->samp_prefix.ensure(nsc * sizeof(double)));
+    HIPCALL(c->samp_rand.ensure(static_cast<size_t>(shots) * sizeof(double)));
+    HIPCALL(c->samp_out.ensure(static_cast<size_t>(shots) * sizeof(int64_t)));
+    HIPCALL(hipMemcpyAsync(c->samp_rand.p, randnums, static_cast<size_t>(shots) * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+    HIPCALL(bk::launch_sample(c->psi.p, c->n, c->precision, c->samp_rand.as<double>(), static_cast<int>(shots),
+                              c->samp_out.as<int64_t>(), c->samp_sums.as<double>(), c->samp_prefix.as<double>(),
+                              c->stream));
+    HIPCALL(hipMemcpyAsync(out_idx, c->samp_out.p, static_cast<size_t>(shots) * sizeof(int64_t),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    return BK_OK;
+}
+
+int bk_hv(bk_ctx* c, int n, double g, int normalize_by_n, int is_complex, const double* d_x, double* d_y,
+          void* stream)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 1 || n > BK_MAX_N || !d_x || !d_y || d_x == d_y || !is_fin(g))
+        return fail(BK_E_INVALID, "bk_hv: bad argument");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    HIPCALL(bk::launch_hv(n, g, normalize_by_n, is_complex, d_x, d_y, st));
+    HIPCALL(hipStreamSynchronize(st));
+    return BK_OK;
+}
+
+int bk_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, int max_krylov_dim, double tol, int k_pairs,
+               double* evals, double* evecs)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 1 || n > 30 || max_krylov_dim < 1 || k_pairs < 1 || !evals || !is_fin(g))
+        return fail(BK_E_INVALID, "bk_lanczos: bad argument");
+    const int64_t dim = 1ll << n;
+    const int m = static_cast<int>(std::min<int64_t>(dim, max_krylov_dim));
+    DevBuf V, w;
+    HIPCALL(V.ensure(sizeof(double) * static_cast<size_t>(dim) * m));
+    HIPCALL(w.ensure(sizeof(double) * static_cast<size_t>(dim)));
+    HIPCALL(c->blas_partials.ensure(sizeof(double) * bk::blas_partials_doubles()));
+    HIPCALL(c->blas_out.ensure(sizeof(double) * 4));
+    double* dV = V.as<double>();
+    double* dw = w.as<double>();
+    auto dot = [&](const double* x, const double* y, double* r) -> int {
+        HIPCALL(bk::launch_dot(dim, 0, x, y, c->blas_out.as<double>(), c->blas_partials.as<double>(), c->stream));
+        double h[2];
+        HIPCALL(hipMemcpyAsync(h, c->blas_out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
+        *r = h[0];
+        return BK_OK;
+    };
+    // v0: mt19937_64(42), uniform(-0.5, 0.5)  (diagonalizer.cuh:441-446)
+    {
+        std::mt19937_64 rng(42);
+        std::uniform_real_distribution<double> dist(-0.5, 0.5);
+        std::vector<double> v0(static_cast<size_t>(dim));
+        for (auto& x : v0) x = dist(rng);
+        HIPCALL(hipMemcpy(dV, v0.data(), v0.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    double nrm2 = 0.0;
+    if ((rc = dot(dV, dV, &nrm2))) return rc;
+    HIPCALL(bk::launch_scal(dim, 1.0 / std::sqrt(nrm2), dV, c->stream));
+    std::vector<double> alpha(m, 0.0), beta(m, 0.0);
+    int m_act = m;
+    for (int j = 0; j < m; ++j) {
+        double* vj = dV + static_cast<size_t>(j) * dim;
+        HIPCALL(bk::launch_hv(n, g, normalize_by_n, 0, vj, dw, c->stream));
+        if (j > 0) HIPCALL(bk::launch_axpy(dim, 0, -beta[j - 1], 0.0, vj - dim, dw, c->stream));
+        if ((rc = dot(vj, dw, &alpha[j]))) return rc;
+        HIPCALL(bk::launch_axpy(dim, 0, -alpha[j], 0.0, vj, dw, c->stream));
+        if (j < m - 1) {
+            double b2 = 0.0;
+            if ((rc = dot(dw, dw, &b2))) return rc;
+            beta[j] = std::sqrt(b2);
+            if (std::fabs(beta[j]) < tol) {
+                m_act = j + 1;
+                break;
+            }
+            double* vn = vj + dim;
+            HIPCALL(hipMemcpyAsync(vn, dw, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->stream));
+            HIPCALL(bk::launch_scal(dim, 1.0 / beta[j], vn, c->stream));
+        }
+    }
+    std::vector<double> d(alpha.begin(), alpha.begin() + m_act);
+    std::vector<double> e(beta.begin(), beta.begin() + std::max(0, m_act - 1));
+    std::vector<double> z;
+    if (!bk::tridiag_eigen(d, e, &z)) return fail(BK_E_INVALID, "tridiagonal eigensolver did not converge");
+    const int kp = std::min(k_pairs, m_act);
+    for (int k = 0; k < kp; ++k) evals[k] = d[k];
+    if (evecs) {
+        DevBuf coef;
+        HIPCALL(coef.ensure(sizeof(double) * m_act));
+        for (int k = 0; k < kp; ++k) {
+            HIPCALL(hipMemcpy(coef.p, z.data() + static_cast<size_t>(k) * m_act, sizeof(double) * m_act,
+                              hipMemcpyHostToDevice));
+            HIPCALL(bk::launch_combine(dim, 0, m_act, dV, coef.as<double>(), dw, c->stream));
+            HIPCALL(hipMemcpyAsync(evecs + static_cast<size_t>(k) * dim, dw, sizeof(double) * dim,
+                                   hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCALL(hipStreamSynchronize(c->stream));
+        coef.release();
+    }
+    V.release();
+    w.release();
+    return kp;
+}
+
+int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, double tau_restart, int m)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
+    if (c->precision != BK_C128) return fail(BK_E_INVALID, "Krylov propagation needs a complex128 state");
+    if (m < 2 || !is_fin(g) || !is_fin(tau) || !(tau_restart > 0.0)) return fail(BK_E_INVALID, "bad argument");
+    if ((rc = flush_pending(c))) return rc;
+    const int n = c->n;
+    const int64_t dim = 1ll << n;
+    DevBuf V, w, coef;
+    HIPCALL(V.ensure(sizeof(double) * 2 * static_cast<size_t>(dim) * m));
+    HIPCALL(w.ensure(sizeof(double) * 2 * static_cast<size_t>(dim)));
+    HIPCALL(coef.ensure(sizeof(double) * 2 * m));
+    HIPCALL(c->blas_partials.ensure(sizeof(double) * bk::blas_partials_doubles()));
+    HIPCALL(c->blas_out.ensure(sizeof(double) * 4));
+    double* dV = V.as<double>();
+    double* dw = w.as<double>();
+    double* psi = c->psi.as<double>();
+    auto zdot = [&](const double* x, const double* y, double* re, double* im) -> int {
+        HIPCALL(bk::launch_dot(dim, 1, x, y, c->blas_out.as<double>(), c->blas_partials.as<double>(), c->stream));
+        double h[2];
+        HIPCALL(hipMemcpyAsync(h, c->blas_out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
+        *re = h[0];
+        if (im) *im = h[1];
+        return BK_OK;
+    };
+    const int nsub = std::max(1, static_cast<int>(std::ceil(std::fabs(tau) / tau_restart - 1e-12)));
+    const double h = tau / nsub;
+    for (int sub = 0; sub < nsub; ++sub) {
+        double b0sq = 0.0;
+        if ((rc = zdot(psi, psi, &b0sq, nullptr))) return rc;
+        const double b0 = std::sqrt(b0sq);
+        if (b0 == 0.0) return BK_OK;
+        HIPCALL(hipMemcpyAsync(dV, psi, sizeof(double) * 2 * dim, hipMemcpyDeviceToDevice, c->stream));
+        HIPCALL(bk::launch_scal(2 * dim, 1.0 / b0, dV, c->stream));
+        std::vector<double> al(m, 0.0), be(m, 0.0);
+        int mm = m;
+        for (int j = 0; j < m; ++j) {
+            double* vj = dV + static_cast<size_t>(j) * 2 * dim;
+            HIPCALL(bk::launch_hv(n, g, normalize_by_n, 1, vj, dw, c->stream));
+            for (int pass = 0; pass < 2; ++pass)
+                for (int i = 0; i <= j; ++i) {
+                    double* vi = dV + static_cast<size_t>(i) * 2 * dim;
+                    double cr = 0.0, ci = 0.0;
+                    if ((rc = zdot(vi, dw, &cr, &ci))) return rc;
+                    if (i == j) al[j] += cr;
+                    HIPCALL(bk::launch_axpy(dim, 1, -cr, -ci, vi, dw, c->stream));
+                }
+            if (j == m - 1) break;
+            double bsq = 0.0;
+            if ((rc = zdot(dw, dw, &bsq, nullptr))) return rc;
+            const double bj = std::sqrt(bsq);
+            if (bj < 1e-13 * (std::fabs(al[j]) + 1.0)) {
+                mm = j + 1;
+                break;
+            }
+            be[j] = bj;
+            double* vn = vj + 2 * dim;
+            HIPCALL(hipMemcpyAsync(vn, dw, sizeof(double) * 2 * dim, hipMemcpyDeviceToDevice, c->stream));
+            HIPCALL(bk::launch_scal(2 * dim, 1.0 / bj, vn, c->stream));
+        }
+        std::vector<double> d(al.begin(), al.begin() + mm);
+        std::vector<double> e(be.begin(), be.begin() + std::max(0, mm - 1));
+        std::vector<double> z;
+        if (!bk::tridiag_eigen(d, e, &z)) return fail(BK_E_INVALID, "tridiagonal eigensolver did not converge");
+        // y = b0 * S exp(-i L h) S^T e1
+        std::vector<double> y(2 * static_cast<size_t>(mm), 0.0);
+        for (int k = 0; k < mm; ++k) {
+            const double s0k = z[static_cast<size_t>(k) * mm];
+            const double cr = std::cos(-d[k] * h) * s0k * b0, ci = std::sin(-d[k] * h) * s0k * b0;
+            for (int j = 0; j < mm; ++j) {
+                y[2 * j] += z[static_cast<size_t>(k) * mm + j] * cr;
+                y[2 * j + 1] += z[static_cast<size_t>(k) * mm + j] * ci;
+            }
+        }
+        HIPCALL(hipMemcpyAsync(coef.p, y.data(), sizeof(double) * y.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCALL(bk::launch_combine(dim, 1, mm, dV, coef.as<double>(), psi, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
+    }
+    V.release();
+    w.release();
+    coef.release();
+    return BK_OK;
+}
+
+void bk_sampler_config_defaults(bk_sampler_config* cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->n = 16;
+    cfg->g = 0.5;
+    cfg->dt = 0.05;
+    cfg->t_max = 60.0;
+    cfg->num_sample_points = 20;
+    cfg->shots_per_point = 100;
+    cfg->seed = 42;
+    cfg->num_sampling_seeds = 3;
+    cfg->seed_stride = 10007;
+    cfg->t_start = 0.0;
+    cfg->dt_sample = 0.0;
+    cfg->track_p_less = 1;
+    cfg->verbose = 1;
+    cfg->record_trajectory = 0;
+    cfg->dt_trajectory = 0.0;
+    cfg->precision = BK_C128;
+    cfg->order = BK_LIE;
+}
+
+void bk_result_free(bk_sampling_result* r)
+{
+    if (!r) return;
+    delete[] r->unique_basis;
+    delete[] r->seed_stats;
+    delete[] r->time_points;
+    delete[] r->p_less_history;
+    delete[] r->dh_history;
+    delete[] r->samples;
+    delete r;
+}
+
+int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits, bk_sampling_result** out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!cfg || !out) return fail(BK_E_INVALID, "null argument");
+    *out = nullptr;
+    const int n = cfg->n;
+    if (n < 1 || n > BK_MAX_N) return fail(BK_E_INVALID, "n=%d outside [1, %d]", n, BK_MAX_N);
+    if (!is_fin(cfg->g) || !is_fin(cfg->dt) || !(cfg->dt > 0.0) || !is_fin(cfg->t_max) || cfg->t_max < 0.0)
+        return fail(BK_E_INVALID, "non-finite or non-positive g/dt/t_max");
+    const int precision = cfg->precision ? cfg->precision : BK_C128;
+    const int order = cfg->order ? cfg->order : BK_LIE;
+    const uint64_t mask = (1ull << n) - 1ull;
+    const uint64_t start = start_bits & mask;
+
+    std::vector<double> tp, pl, dh;
+    bk_sampling_result* r = new bk_sampling_result();
+    std::memset(r, 0, sizeof(*r));
+    // freed on every early return; released to the caller on success
+    std::unique_ptr<bk_sampling_result, void (*)(bk_sampling_result*)> guard(r, bk_result_free);
+    auto cleanup_fail = [&](int code) { return code; };
+    // sampler.cuh:420-423
+    r->start_energy = bk::host_energy(n, start);
+    r->d_min = bk::host_d_min(n, start, r->start_energy, -1);
+    if ((rc = bk_state_init(c, n, precision, start))) return cleanup_fail(rc);
+    uint64_t cnt = 0;
+    if ((rc = bk_count_below(c, n, r->start_energy, &cnt))) return cleanup_fail(rc);
+    r->p_blind = static_cast<double>(cnt) / static_cast<double>(1ull << n);
+
+    bk::ScheduleIn si{};
+    si.t_max = cfg->t_max;
+    si.dt = cfg->dt;
+    si.t_start = cfg->t_start;
+    si.dt_sample = cfg->dt_sample;
+    si.dt_trajectory = cfg->dt_trajectory;
+    si.num_sample_points = cfg->num_sample_points;
+    si.shots_per_point = cfg->shots_per_point;
+    si.track_p_less = cfg->track_p_less != 0;
+    si.record_trajectory = cfg->record_trajectory != 0;
+    std::vector<unsigned char> flags;
+    const int total = bk::build_schedule(si, flags);
+
+    int n_meas = 0, n_pts = 0;
+    for (unsigned char f : flags) {
+        n_meas += (f & 1) ? 1 : 0;
+        n_pts += (f & 2) ? 1 : 0;
+    }
+    const int shots = std::max(1, cfg->shots_per_point);
+    const int n_seeds = std::max(1, cfg->num_sampling_seeds);
+    // random numbers in the reference's draw order (sampler.cuh:481-487, 567-568)
+    std::vector<std::mt19937_64> rngs(n_seeds);
+    for (int s = 0; s < n_seeds; ++s) rngs[s].seed(cfg->seed + static_cast<uint64_t>(s) * cfg->seed_stride);
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    const size_t per_point = static_cast<size_t>(n_seeds) * shots;
+    std::vector<double> rnd(per_point * static_cast<size_t>(n_pts));
+    for (int ptn = 0; ptn < n_pts; ++ptn)
+        for (int s = 0; s < n_seeds; ++s)
+            for (int i = 0; i < shots; ++i) rnd[ptn * per_point + s * shots + i] = dist(rngs[s]);
+    std::vector<int64_t> idx(rnd.size());
+    if (n_pts > 0) {
+        const size_t nsc = bk::sample_scratch_doubles(n);
+        HIPCALL(c->samp_sums.ensure(nsc * sizeof(double)));
+        HIPCALL(c->samp_prefix.ensure(nsc * sizeof(double)));
+        HIPCALL(c->samp_rand.ensure(rnd.size() * sizeof(double)));
+        HIPCALL(c->samp_out.ensure(rnd.size() * sizeof(int64_t)));
+        HIPCALL(hipMemcpyAsync(c->samp_rand.p, rnd.data(), rnd.size() * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    }
+    HIPCALL(c->series.ensure(3 * sizeof(double) * static_cast<size_t>(std::max(n_meas, 1))));
+    double* ds = c->series.as<double>();
+    const uint32_t e0u = static_cast<uint32_t>(std::max<int64_t>(0, r->start_energy));
+    int mi = 0, pi = 0;
+    for (int step = 0; step <= total; ++step) {
+        const unsigned char f = flags[static_cast<size_t>(step)];
+        if (f & 1) tp.push_back(step * cfg->dt);
+        if (f & 2) {  // sampling reads |psi|^2 of the state before this step
+            for (int s = 0; s < n_seeds; ++s) {
+                const size_t off = pi * per_point + static_cast<size_t>(s) * shots;
+                HIPCALL(bk::launch_sample(c->psi.p, n, precision, c->samp_rand.as<double>() + off, shots,
+                                          c->samp_out.as<int64_t>() + off, c->samp_sums.as<double>(),
+                                          c->samp_prefix.as<double>(), c->stream));
+            }
+            ++pi;
+        }
+        double* slot = (f & 1) ? ds + 3 * (mi++) : nullptr;
+        if (step < total) {
+            if ((rc = enqueue_step(c, order, cfg->dt, cfg->g, start, e0u, slot))) return cleanup_fail(rc);
+        } else if (slot) {
+            HIPCALL(bk::launch_measure(c->psi.p, n, precision, static_cast<uint32_t>(start), e0u, slot,
+                                       c->partials.as<double>(), c->tuning.grid, c->stream));
+        }
+    }
+    std::vector<double> hs(3 * static_cast<size_t>(n_meas));
+    if (n_meas > 0)
+        HIPCALL(hipMemcpyAsync(hs.data(), ds, hs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (n_pts > 0)
+        HIPCALL(hipMemcpyAsync(idx.data(), c->samp_out.p, idx.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n_meas; ++i) {
+        pl.push_back(hs[3 * i]);
+        dh.push_back(hs[3 * i + 1]);
+    }
+    if (cfg->verbose)
+        for (size_t i = 0; i < tp.size(); ++i) {
+            const int step = static_cast<int>(std::llround(tp[i] / cfg->dt));
+            if (!cfg->record_trajectory || !(step % 200 == 0 || step == total)) continue;
+            std::cout << "    [Progress " << std::setw(3)
+                      << static_cast<int>(total > 0 ? 100.0 * step / total : 100.0) << "%] Step " << std::setw(4)
+                      << step << "/" << total << " (t = " << std::fixed << std::setprecision(1) << tp[i] << "s)"
+                      << " | P_<(t) = " << std::scientific << std::setprecision(3) << pl[i]
+                      << " | <dH> = " << std::fixed << std::setprecision(2) << dh[i] << std::defaultfloat
+                      << std::endl;
+        }
+
+    // per-shot statistics (sampler.cuh:575-597, 622-640)
+    std::set<uint64_t> uniq;
+    std::vector<bk_seed_stats> ss(n_seeds);
+    for (int s = 0; s < n_seeds; ++s) {
+        std::memset(&ss[s], 0, sizeof(bk_seed_stats));
+        ss[s].seed = cfg->seed + static_cast<uint64_t>(s) * cfg->seed_stride;
+        ss[s].best_energy = r->start_energy;
+    }
+    uint64_t total_flips = 0;
+    std::vector<double> seed_flips(n_seeds, 0.0);
+    for (int ptn = 0; ptn < n_pts; ++ptn) {
+        for (int s = 0; s < n_seeds; ++s) {
+            uint64_t flips_this = 0;
+            for (int i = 0; i < shots; ++i) {
+                const uint64_t b = static_cast<uint64_t>(idx[ptn * per_point + static_cast<size_t>(s) * shots + i]);
+                uniq.insert(b);
+                const int d = __builtin_popcountll(start ^ b);
+                flips_this += d;
+                if (s == 0) total_flips += d;
+                const int64_t e = bk::host_energy(n, b);
+                if (e < ss[s].best_energy) ss[s].best_energy = e;
+                if (e < r->start_energy) {
+                    ss[s].improved_shots++;
+                    if (s == 0) r->improved_shots++;
+                }
+            }
+            ss[s].total_shots_collected += shots;
+            seed_flips[s] += static_cast<double>(flips_this);
+        }
+        r->total_shots_collected += shots;
+    }
+    r->total_evals = r->total_shots_collected;
+    if (r->total_shots_collected > 0) {
+        r->mean_hamming_flips = static_cast<double>(total_flips) / static_cast<double>(r->total_shots_collected);
+        r->success_rate = static_cast<double>(r->improved_shots) / static_cast<double>(r->total_shots_collected);
+        r->per_eval_rate = r->total_evals > 0
+                               ? static_cast<double>(r->improved_shots) / static_cast<double>(r->total_evals)
+                               : 0.0;
+    }
+    for (int s = 0; s < n_seeds; ++s) {
+        ss[s].mean_hamming_flips = seed_flips[s];
+        if (ss[s].total_shots_collected > 0) {
+            ss[s].success_rate =
+                static_cast<double>(ss[s].improved_shots) / static_cast<double>(ss[s].total_shots_collected);
+            ss[s].per_eval_rate = ss[s].success_rate;
+            ss[s].mean_hamming_flips /= static_cast<double>(ss[s].total_shots_collected);
+        }
+    }
+    const double avg_end = (cfg->num_sample_points > 1 && cfg->dt_sample > 0.0)
+                               ? cfg->t_start + (cfg->num_sample_points - 1) * cfg->dt_sample
+                               : cfg->t_max;
+    const bk::Derived dm = bk::derived_metrics(tp, pl, dh, cfg->t_start, avg_end);
+    r->initial_rise_rate_avg = dm.rise_avg;
+    r->initial_rise_rate_max = dm.rise_max;
+    r->peak_p_less = dm.peak_p;
+    r->peak_time = dm.peak_t;
+    r->window_avg_dh = dm.window_dh;
+
+    r->history_len = static_cast<int64_t>(tp.size());
+    r->time_points = new double[tp.size() + 1];
+    r->p_less_history = new double[tp.size() + 1];
+    r->dh_history = new double[tp.size() + 1];
+    std::copy(tp.begin(), tp.end(), r->time_points);
+    std::copy(pl.begin(), pl.end(), r->p_less_history);
+    std::copy(dh.begin(), dh.end(), r->dh_history);
+    r->unique_basis_len = static_cast<int64_t>(uniq.size());
+    r->unique_basis = new uint64_t[uniq.size() + 1];
+    std::copy(uniq.begin(), uniq.end(), r->unique_basis);
+    r->seed_stats_len = n_seeds;
+    r->seed_stats = new bk_seed_stats[n_seeds];
+    std::copy(ss.begin(), ss.end(), r->seed_stats);
+    r->samples_len = static_cast<int64_t>(idx.size());
+    r->samples = new int64_t[idx.size() + 1];
+    std::copy(idx.begin(), idx.end(), r->samples);
+    if (cfg->verbose) {
+        std::cout << "[Quantum Sampler] Successes / Trials: " << r->improved_shots << " / "
+                  << r->total_shots_collected << " (" << std::fixed << std::setprecision(2)
+                  << (r->success_rate * 100.0) << "%)"
+                  << " | E < E_og (E_og = " << r->start_energy << ")"
+                  << " | Window <dH>: " << std::fixed << std::setprecision(2) << r->window_avg_dh
+                  << " | Rise Rate (avg): " << std::scientific << std::setprecision(3) << r->initial_rise_rate_avg
+                  << " | Evals: " << r->total_evals << " | Per-Eval Rate: " << std::scientific
+                  << std::setprecision(4) << r->per_eval_rate << std::defaultfloat << "\n";
+    }
+    *out = guard.release();
+    return BK_OK;
+}
+
+int bk_timing_enable(bk_ctx* c, int enable)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    HIPCALL(hipStreamSynchronize(c->stream));
+    c->timing = enable != 0;
+    c->timer.reset();
+    c->tuning.timer = c->timing ? &c->timer : nullptr;
+    return BK_OK;
+}
+
+int bk_timing_read(bk_ctx* c, bk_kernel_timing* out, int cap, int* count)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!count || (cap > 0 && !out)) return fail(BK_E_INVALID, "bk_timing_read: bad argument");
+    HIPCALL(hipStreamSynchronize(c->stream));
+    if (c->timer.err != hipSuccess) {
+        hipError_t e = c->timer.err;
+        c->timer.reset();
+        return fail(BK_E_HIP, "timing events: %s", hipGetErrorString(e));
+    }
+    std::vector<bk_kernel_timing> rows;
+    for (const auto& r : c->timer.recs) {
+        float ms = 0.0f;
+        HIPCALL(hipEventElapsedTime(&ms, c->timer.pool[r.ev0], c->timer.pool[r.ev1]));
+        auto it = std::find_if(rows.begin(), rows.end(),
+                               [&](const bk_kernel_timing& k) { return r.name == k.name; });
+        if (it == rows.end()) {
+            bk_kernel_timing k{};
+            std::snprintf(k.name, sizeof(k.name), "%s", r.name.c_str());
+            k.bytes_per_launch = r.bytes;
+            rows.push_back(k);
+            it = rows.end() - 1;
+        }
+        it->launches += 1;
+        it->total_ms += ms;
+    }
+    c->timer.reset();
+    *count = static_cast<int>(rows.size());
+    for (int i = 0; i < cap && i < static_cast<int>(rows.size()); ++i) out[i] = rows[i];
+    return BK_OK;
+}
+
+}  // extern "C"

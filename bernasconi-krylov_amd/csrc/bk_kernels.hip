@@ -1,0 +1,645 @@
+// bk_kernels.hip -- gfx950 kernels for the Trotterised transverse-field evolution
+// under H = E(s)/N - g sum_i sigma^x_i (sampler.cuh:54-227, 604-617) and the
+// matrix-free H.v of diagonalizer.cuh:60-77.
+//
+// Hot path (one Trotter step, bandwidth-bound):
+//   LOW sweep   : one HBM pass over contiguous 2^TB-amplitude tiles.  Per
+//                 amplitude it computes E(s) with popcounts (labs_energy.hpp:
+//                 121-132), accumulates P_<, <d_H> and the norm of the incoming
+//                 state in FP64 (sampler.cuh:149-195), multiplies by the
+//                 tabulated phase exp(-i dt E/N) (sampler.cuh:73-91), then
+//                 applies the sigma_x rotations of qubits 0..TB-1 (registers +
+//                 LDS exchanges).
+//   HIGH sweeps : one HBM pass each, rotating the next TB-R qubits on tiles
+//                 made of 2^R-amplitude contiguous rows.
+// The per-qubit rotation [[c, i s],[i s, c]] (sampler.cuh:451-457) is applied
+// as c * [[1, i t],[i t, 1]] with t = tan(g dt): every butterfly is 4 FMAs and
+// the product of the n factors c is folded into the phase multiply.  When
+// |cos(g dt)| < 0.5 the full (c, s) butterfly is used instead.
+#include "bk_kernels.hpp"
+#include "bk_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+namespace bk {
+
+template <typename V>
+struct RealOf;
+template <>
+struct RealOf<float2> {
+    using T = float;
+};
+template <>
+struct RealOf<double2> {
+    using T = double;
+};
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+// labs_energy.hpp:121-132 on 32-bit words (n <= 32).
+__device__ __forceinline__ uint32_t labs_energy_u32(uint32_t s, int n)
+{
+    uint32_t e = 0;
+#pragma unroll 4
+    for (int k = 1; k < n; ++k) {
+        const uint32_t pairs = (s ^ (s >> k)) & (0xFFFFFFFFu >> (32 - (n - k)));
+        const int c = (n - k) - 2 * static_cast<int>(__popc(pairs));
+        e += static_cast<uint32_t>(c * c);
+    }
+    return e;
+}
+
+template <typename V, typename R>
+__device__ __forceinline__ void bfly_tan(V& a, V& b, R t)
+{
+    const R ax = a.x, ay = a.y;
+    a.x = fma(-t, b.y, ax);
+    a.y = fma(t, b.x, ay);
+    b.x = fma(-t, ay, b.x);
+    b.y = fma(t, ax, b.y);
+}
+
+template <typename V, typename R>
+__device__ __forceinline__ void bfly_cs(V& a, V& b, R c, R s)
+{
+    const R ax = a.x, ay = a.y;
+    a.x = c * ax - s * b.y;
+    a.y = c * ay + s * b.x;
+    b.x = c * b.x - s * ay;
+    b.y = c * b.y + s * ax;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Deterministic block reduction of NV doubles (256 threads = 4 waves).
+// 'scratch' must hold 4*NV doubles and be free for use.
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* scratch, double* out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) scratch[wave * NV + i] = v[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            out[i] = (scratch[0 * NV + i] + scratch[1 * NV + i]) +
+                     (scratch[2 * NV + i] + scratch[3 * NV + i]);
+    }
+}
+
+// LDS swizzle: XOR the low SW bits of the element index with the next SW bits.
+template <int SW>
+__device__ __forceinline__ uint32_t swz(uint32_t u)
+{
+    return u ^ ((u >> SW) & ((1u << SW) - 1u));
+}
+
+// ---------------------------------------------------------------------------
+// Tiled sweep kernel
+// ---------------------------------------------------------------------------
+enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4 };
+
+struct SweepArgs {
+    uint64_t ntiles;
+    const void* tab;   // phase table (V per energy) for F_PHASE
+    double* partials;  // gridDim.x * 3 doubles for F_MEASURE
+    double bc, bs;     // butterfly coefficients (F_TAN: bs = tan, bc unused)
+    double scale;      // overall factor applied with the phase (c^n in F_TAN form)
+    int n;
+    int h;             // HIGH: global bit of local bit R
+    int lo_bits;       // HIGH: tile-index bits placed at global [R, h)
+    uint32_t s0, e0;
+};
+
+template <int TB_, int RB_, int R_>
+struct HighPlan {
+    static constexpr Plan plan = make_high_plan(TB_, RB_, R_);
+};
+template <int TB_, int RB_>
+struct LowPlan {
+    static constexpr Plan plan = make_low_plan(TB_, RB_);
+};
+
+template <typename V, class PH, int FLAGS, int SW>
+__global__ __launch_bounds__(kThreads) void k_sweep(V* __restrict__ psi, const SweepArgs a)
+{
+    using R = typename RealOf<V>::T;
+    constexpr Plan P = PH::plan;
+    constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R;
+    constexpr bool LOW = P.low;
+    constexpr uint32_t ROWMASK = (1u << RR) - 1u;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* lds = reinterpret_cast<V*>(smem_raw);
+    const uint32_t t = threadIdx.x;
+
+    uint32_t tu[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        uint32_t u = 0;
+#pragma unroll
+        for (int b = 0; b < kThreadBits; ++b) u |= ((t >> b) & 1u) << P.ph[p].thr[b];
+        tu[p] = u;
+    }
+    // global offset of a local index (HIGH: rows stay, active bits move to h)
+    auto gl = [&](uint32_t u) -> uint64_t {
+        if constexpr (LOW) return u;
+        else return static_cast<uint64_t>(u & ROWMASK) + (static_cast<uint64_t>(u >> RR) << a.h);
+    };
+    const uint64_t tg0 = gl(tu[0]);
+    const uint64_t tgL = gl(tu[NP - 1]);
+    const R bc = static_cast<R>(a.bc), bs = static_cast<R>(a.bs);
+    double acc[3] = {0.0, 0.0, 0.0};
+
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        uint64_t base;
+        if constexpr (LOW) {
+            base = tile << TB;
+        } else {
+            const uint64_t lo = tile & ((1ull << a.lo_bits) - 1ull);
+            const uint64_t hi = tile >> a.lo_bits;
+            base = (lo << RR) | (hi << (a.h + TB - RR));
+        }
+        V x[NR];
+        V* tp = psi + base;
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const uint32_t pj = reg_pattern(P.ph[0], RB, j);
+            x[j] = tp[tg0 + gl(pj)];
+        }
+        if constexpr ((FLAGS & (F_PHASE | F_MEASURE)) != 0) {
+            static_assert(LOW, "phase/measure only in the LOW sweep");
+            const V* tab = static_cast<const V*>(a.tab);
+            const R sc = static_cast<R>(a.scale);
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                const uint32_t s = static_cast<uint32_t>(base + tg0 + reg_pattern(P.ph[0], RB, j));
+                const uint32_t E = labs_energy_u32(s, a.n);
+                if constexpr ((FLAGS & F_MEASURE) != 0) {
+                    const double xr = static_cast<double>(x[j].x), xi = static_cast<double>(x[j].y);
+                    const double pr = fma(xr, xr, xi * xi);
+                    acc[2] += pr;
+                    acc[0] += (E < a.e0) ? pr : 0.0;
+                    acc[1] = fma(pr, static_cast<double>(__popc(s ^ a.s0)), acc[1]);
+                }
+                if constexpr ((FLAGS & F_PHASE) != 0) {
+                    const V ph = tab[E];
+                    const R pr = ph.x * sc, pi = ph.y * sc;
+                    const R xr = x[j].x, xi = x[j].y;
+                    x[j].x = xr * pr - xi * pi;
+                    x[j].y = xr * pi + xi * pr;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            if (p > 0) {
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < NR; ++j)
+                    lds[swz<SW>(tu[p - 1] | reg_pattern(P.ph[p - 1], RB, j))] = x[j];
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < NR; ++j) x[j] = lds[swz<SW>(tu[p] | reg_pattern(P.ph[p], RB, j))];
+            }
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                if (!((P.ph[p].act >> b) & 1u)) continue;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    if ((j >> b) & 1) continue;
+                    if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[j], x[j | (1 << b)], bs);
+                    else bfly_cs<V, R>(x[j], x[j | (1 << b)], bc, bs);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const uint32_t pj = reg_pattern(P.ph[NP - 1], RB, j);
+            tp[tgL + gl(pj)] = x[j];
+        }
+    }
+    if constexpr ((FLAGS & F_MEASURE) != 0) {
+        __syncthreads();
+        block_reduce_store<3>(acc, reinterpret_cast<double*>(smem_raw), a.partials + 3 * blockIdx.x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Small-n / generic kernels
+// ---------------------------------------------------------------------------
+
+// one qubit's rotation over all pairs (used when n is below the tile size)
+template <typename V, bool TAN>
+__global__ __launch_bounds__(kThreads) void k_qubit(V* __restrict__ psi, int q, uint64_t npairs,
+                                                   double bc_, double bs_)
+{
+    using R = typename RealOf<V>::T;
+    const R bc = static_cast<R>(bc_), bs = static_cast<R>(bs_);
+    const uint64_t st = 1ull << q;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < npairs;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t lo = ((i >> q) << (q + 1)) | (i & (st - 1));
+        V x0 = psi[lo], x1 = psi[lo | st];
+        if (TAN) bfly_tan<V, R>(x0, x1, bs);
+        else bfly_cs<V, R>(x0, x1, bc, bs);
+        psi[lo] = x0;
+        psi[lo | st] = x1;
+    }
+}
+
+// elementwise measurement (F_MEASURE) and/or phase*scale (F_PHASE) over the
+// whole state; contiguous chunk per block so the partials are deterministic.
+template <typename V, int FLAGS>
+__global__ __launch_bounds__(kThreads) void k_elementwise(V* __restrict__ psi, uint64_t dim,
+                                                         uint64_t chunk, const void* tab_,
+                                                         double scale, int n, uint32_t s0,
+                                                         uint32_t e0, double* partials)
+{
+    using R = typename RealOf<V>::T;
+    __shared__ double scratch[12];
+    const V* tab = static_cast<const V*>(tab_);
+    const R sc = static_cast<R>(scale);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const uint64_t beg = blockIdx.x * chunk;
+    const uint64_t end = (beg + chunk < dim) ? beg + chunk : dim;
+    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        V x = psi[i];
+        const uint32_t s = static_cast<uint32_t>(i);
+        const uint32_t E = labs_energy_u32(s, n);
+        if constexpr ((FLAGS & F_MEASURE) != 0) {
+            const double xr = static_cast<double>(x.x), xi = static_cast<double>(x.y);
+            const double pr = fma(xr, xr, xi * xi);
+            acc[2] += pr;
+            acc[0] += (E < e0) ? pr : 0.0;
+            acc[1] = fma(pr, static_cast<double>(__popc(s ^ s0)), acc[1]);
+        }
+        if constexpr ((FLAGS & F_PHASE) != 0) {
+            const V ph = tab[E];
+            const R pr = ph.x * sc, pi = ph.y * sc;
+            const R xr = x.x, xi = x.y;
+            x.x = xr * pr - xi * pi;
+            x.y = xr * pi + xi * pr;
+            psi[i] = x;
+        }
+    }
+    if constexpr ((FLAGS & F_MEASURE) != 0) block_reduce_store<3>(acc, scratch, partials + 3 * blockIdx.x);
+}
+
+// sum 'count' triples of partials in a fixed order into out[0..2]
+__global__ __launch_bounds__(kThreads) void k_finish3(const double* __restrict__ partials, int count,
+                                                     double* __restrict__ out)
+{
+    __shared__ double scratch[12];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        acc[0] += partials[3 * i + 0];
+        acc[1] += partials[3 * i + 1];
+        acc[2] += partials[3 * i + 2];
+    }
+    block_reduce_store<3>(acc, scratch, out);
+}
+
+template <typename V>
+__global__ void k_set_basis(V* psi, uint64_t s0)
+{
+    V one;
+    one.x = 1;
+    one.y = 0;
+    psi[s0] = one;
+}
+
+// number of s in [0, 2^n) with E(s) < e0
+__global__ __launch_bounds__(kThreads) void k_count_below(int n, uint32_t e0, uint64_t dim,
+                                                         uint64_t chunk,
+                                                         unsigned long long* partials)
+{
+    __shared__ unsigned long long scratch[4];
+    unsigned long long c = 0;
+    const uint64_t beg = blockIdx.x * chunk;
+    const uint64_t end = (beg + chunk < dim) ? beg + chunk : dim;
+    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x)
+        c += (labs_energy_u32(static_cast<uint32_t>(i), n) < e0) ? 1ull : 0ull;
+    c = wave_sum_u64(c);
+    if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_finish_u64(const unsigned long long* __restrict__ partials,
+                                                        int count, unsigned long long* out)
+{
+    __shared__ unsigned long long scratch[4];
+    unsigned long long c = 0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) c += partials[i];
+    c = wave_sum_u64(c);
+    if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kLowTB64 = 13, kLowRB64 = 5;
+constexpr int kLowTB128 = 12, kLowRB128 = 4;
+constexpr int kSW64 = 5, kSW128 = 4;
+
+template <typename V>
+struct Cfg;
+template <>
+struct Cfg<float2> {
+    static constexpr int TB = kLowTB64, RB = kLowRB64, SW = kSW64;
+};
+template <>
+struct Cfg<double2> {
+    static constexpr int TB = kLowTB128, RB = kLowRB128, SW = kSW128;
+};
+
+struct MixCoef {
+    bool tan_form;
+    double bc, bs, scale;
+};
+
+MixCoef mix_coef(int n, double gdt)
+{
+    // reference matrix [[cos(rx/2), -i sin(rx/2)], ...] with rx = -2 g dt
+    const double rx = -2.0 * gdt;
+    const double c = std::cos(rx / 2.0), s = -std::sin(rx / 2.0);
+    MixCoef m{};
+    if (std::fabs(c) >= 0.5) {
+        m.tan_form = true;
+        m.bc = 1.0;
+        m.bs = s / c;
+        m.scale = std::pow(c, n);
+    } else {
+        m.tan_form = false;
+        m.bc = c;
+        m.bs = s;
+        m.scale = 1.0;
+    }
+    return m;
+}
+
+template <typename K>
+hipError_t set_lds(K kernel, size_t bytes)
+{
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+}
+
+template <typename V, int FLAGS>
+hipError_t launch_low(V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FLAGS, C::SW>;
+    const size_t lds = sizeof(V) << C::TB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    return hipGetLastError();
+}
+
+template <typename V, int R, bool TAN>
+hipError_t launch_high_r(V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    auto k = k_sweep<V, HighPlan<C::TB, C::RB, R>, TAN ? F_TAN : 0, C::SW>;
+    const size_t lds = sizeof(V) << C::TB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    return hipGetLastError();
+}
+
+template <typename V, bool TAN, int R = 3>
+hipError_t launch_high(V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (R > Cfg<V>::TB - 1) {
+        return hipErrorInvalidValue;
+    } else {
+        if (rbits == R) return launch_high_r<V, R, TAN>(psi, a, grid, st);
+        return launch_high<V, TAN, R + 1>(psi, rbits, a, grid, st);
+    }
+}
+
+int clamp_grid(uint64_t ntiles, int grid)
+{
+    const uint64_t g = std::min<uint64_t>(ntiles, static_cast<uint64_t>(std::max(grid, 1)));
+    return static_cast<int>(g);
+}
+
+template <typename V>
+hipError_t step_impl(V* psi, const StepSpec& sp, const SweepTuning& tun, double* partials,
+                     hipStream_t st)
+{
+    using C = Cfg<V>;
+    const int n = sp.n;
+    const MixCoef mc = mix_coef(n, sp.gdt);
+    const uint64_t dim = 1ull << n;
+    const bool measure = sp.series_slot != nullptr;
+    hipError_t e = hipSuccess;
+    if (n < C::TB) {
+        // small states: elementwise phase (+measure) then one kernel per qubit
+        const uint64_t chunk = 4096;
+        const int grid = static_cast<int>((dim + chunk - 1) / chunk);
+        if (measure)
+            hipLaunchKernelGGL((k_elementwise<V, F_PHASE | F_MEASURE>), dim3(grid), dim3(kThreads), 0,
+                               st, psi, dim, chunk, sp.phase_tab, mc.scale, n, sp.s0, sp.e0, partials);
+        else
+            hipLaunchKernelGGL((k_elementwise<V, F_PHASE>), dim3(grid), dim3(kThreads), 0, st, psi,
+                               dim, chunk, sp.phase_tab, mc.scale, n, sp.s0, sp.e0, partials);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (measure) {
+            hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, grid, sp.series_slot);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        const uint64_t np = dim / 2;
+        const int qg = static_cast<int>(std::min<uint64_t>((np + kThreads - 1) / kThreads, 4096));
+        for (int q = 0; q < n; ++q) {
+            if (mc.tan_form)
+                hipLaunchKernelGGL((k_qubit<V, true>), dim3(qg), dim3(kThreads), 0, st, psi, q, np,
+                                   mc.bc, mc.bs);
+            else
+                hipLaunchKernelGGL((k_qubit<V, false>), dim3(qg), dim3(kThreads), 0, st, psi, q, np,
+                                   mc.bc, mc.bs);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    SweepArgs a{};
+    a.tab = sp.phase_tab;
+    a.partials = partials;
+    a.bc = mc.bc;
+    a.bs = mc.bs;
+    a.scale = mc.scale;
+    a.n = n;
+    a.s0 = sp.s0;
+    a.e0 = sp.e0;
+    // LOW sweep: phase (+ measurement) and qubits [0, TB)
+    a.ntiles = dim >> C::TB;
+    const int g1 = clamp_grid(a.ntiles, tun.grid);
+    const double sweep_bytes = 2.0 * static_cast<double>(sizeof(V)) * static_cast<double>(dim);
+    if (tun.timer) tun.timer->begin(st);
+    if (measure) {
+        e = mc.tan_form ? launch_low<V, F_PHASE | F_MEASURE | F_TAN>(psi, a, g1, st)
+                        : launch_low<V, F_PHASE | F_MEASURE>(psi, a, g1, st);
+    } else {
+        e = mc.tan_form ? launch_low<V, F_PHASE | F_TAN>(psi, a, g1, st)
+                        : launch_low<V, F_PHASE>(psi, a, g1, st);
+    }
+    if (e != hipSuccess) return e;
+    if (tun.timer) tun.timer->end(st, "sweep_low", sweep_bytes);
+    if (measure) {
+        hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, g1, sp.series_slot);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    // HIGH sweeps over the remaining qubits [TB, n)
+    const int rmin = std::max(tun.row_bits, 3);
+    const int amax = C::TB - rmin;
+    const int remaining = n - C::TB;
+    if (remaining <= 0) return hipSuccess;
+    const int nh = (remaining + amax - 1) / amax;
+    int h = C::TB;
+    for (int i = 0; i < nh; ++i) {
+        const int left = n - h;
+        const int A = (left + (nh - i) - 1) / (nh - i);  // even split, larger first
+        const int R = C::TB - A;
+        SweepArgs b = a;
+        b.h = h;
+        b.lo_bits = h - R;
+        b.ntiles = dim >> C::TB;
+        const int g = clamp_grid(b.ntiles, tun.grid);
+        if (tun.timer) tun.timer->begin(st);
+        e = mc.tan_form ? launch_high<V, true>(psi, R, b, g, st) : launch_high<V, false>(psi, R, b, g, st);
+        if (e != hipSuccess) return e;
+        if (tun.timer) {
+            char name[48];
+            std::snprintf(name, sizeof(name), "sweep_high_q%d_%d", h, h + A - 1);
+            tun.timer->end(st, name, sweep_bytes);
+        }
+        h += A;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+SweepTuning default_tuning(int n, int precision)
+{
+    (void)n;
+    SweepTuning t{};
+    t.tile_bits = precision == kC64 ? kLowTB64 : kLowTB128;
+    t.row_bits = precision == kC64 ? 4 : 3;
+    t.grid = 2048;
+    return t;
+}
+
+size_t partials_bytes(int grid) { return sizeof(double) * 3 * static_cast<size_t>(std::max(grid, 4096)); }
+
+hipError_t launch_step(void* psi, const StepSpec& spec, const SweepTuning& tun, double* partials,
+                       hipStream_t stream)
+{
+    if (spec.n < 1 || spec.n > 32) return hipErrorInvalidValue;
+    if (spec.precision == kC64) return step_impl(static_cast<float2*>(psi), spec, tun, partials, stream);
+    if (spec.precision == kC128) return step_impl(static_cast<double2*>(psi), spec, tun, partials, stream);
+    return hipErrorInvalidValue;
+}
+
+template <typename V>
+static hipError_t measure_impl(V* psi, int n, uint32_t s0, uint32_t e0, double* slot, double* partials,
+                               hipStream_t st)
+{
+    const uint64_t dim = 1ull << n;
+    uint64_t chunk = 4096;
+    while ((dim + chunk - 1) / chunk > 4096) chunk <<= 1;
+    const int grid = static_cast<int>((dim + chunk - 1) / chunk);
+    hipLaunchKernelGGL((k_elementwise<V, F_MEASURE>), dim3(grid), dim3(kThreads), 0, st, psi, dim, chunk,
+                       nullptr, 1.0, n, s0, e0, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, grid, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_measure(const void* psi, int n, int precision, uint32_t s0, uint32_t e0,
+                          double* series_slot, double* partials, int grid, hipStream_t stream)
+{
+    (void)grid;
+    if (precision == kC64)
+        return measure_impl(static_cast<float2*>(const_cast<void*>(psi)), n, s0, e0, series_slot, partials,
+                            stream);
+    if (precision == kC128)
+        return measure_impl(static_cast<double2*>(const_cast<void*>(psi)), n, s0, e0, series_slot, partials,
+                            stream);
+    return hipErrorInvalidValue;
+}
+
+template <typename V>
+static hipError_t phase_impl(V* psi, int n, const void* tab, hipStream_t st)
+{
+    const uint64_t dim = 1ull << n;
+    uint64_t chunk = 4096;
+    while ((dim + chunk - 1) / chunk > 65536) chunk <<= 1;
+    const int grid = static_cast<int>((dim + chunk - 1) / chunk);
+    hipLaunchKernelGGL((k_elementwise<V, F_PHASE>), dim3(grid), dim3(kThreads), 0, st, psi, dim, chunk, tab,
+                       1.0, n, 0u, 0u, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, hipStream_t stream)
+{
+    if (precision == kC64) return phase_impl(static_cast<float2*>(psi), n, phase_tab, stream);
+    if (precision == kC128) return phase_impl(static_cast<double2*>(psi), n, phase_tab, stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, hipStream_t stream)
+{
+    const size_t esz = precision == kC64 ? sizeof(float2) : sizeof(double2);
+    hipError_t e = hipMemsetAsync(psi, 0, esz << n, stream);
+    if (e != hipSuccess) return e;
+    if (precision == kC64) hipLaunchKernelGGL(k_set_basis<float2>, dim3(1), dim3(1), 0, stream,
+                                              static_cast<float2*>(psi), s0);
+    else hipLaunchKernelGGL(k_set_basis<double2>, dim3(1), dim3(1), 0, stream, static_cast<double2*>(psi), s0);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_below(int n, uint32_t e0, unsigned long long* d_count, unsigned long long* partials,
+                              int grid, hipStream_t stream)
+{
+    (void)grid;
+    const uint64_t dim = 1ull << n;
+    uint64_t chunk = 4096;
+    while ((dim + chunk - 1) / chunk > 4096) chunk <<= 1;
+    const int g = static_cast<int>((dim + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_count_below, dim3(g), dim3(kThreads), 0, stream, n, e0, dim, chunk, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finish_u64, dim3(1), dim3(kThreads), 0, stream, partials, g, d_count);
+    return hipGetLastError();
+}
+
+}  // namespace bk

@@ -1,0 +1,98 @@
+// bk_kernels.hpp -- host-side interface of the gfx950 kernels (launchers).
+//
+// Device memory layout: the statevector is one contiguous array indexed by
+// the spin bit pattern s (bit i <-> spin i, bit 1 <-> spin -1;
+// labs_energy.hpp:20-23), amplitudes interleaved (re, im) as float2 (c64) or
+// double2 (c128) -- the layout of sampler.cuh:426 in either precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bk {
+
+enum Precision : int { kC64 = 1, kC128 = 2 };
+
+// One Trotter step's kernel sequence for a statevector of 2^n amplitudes.
+struct StepSpec {
+    int n;
+    int precision;
+    // mixer: per-qubit [[c, i s],[i s, c]], c = cos(g dt), s = sin(g dt)
+    double gdt;
+    // diagonal phase applied in the first sweep before the rotations:
+    // exp(-i dtp E(s)/n); dtp == 0 -> no phase
+    double dtp;
+    const void* phase_tab;  // device table indexed by E (float2 or double2)
+    // measurement of the incoming state (before the phase) into slot
+    double* series_slot;    // device pointer to 3 doubles {P_<, <d_H>, norm} or null
+    uint32_t s0;
+    uint32_t e0;
+};
+
+// Optional per-launch timing hook (HIP events recorded on the launch stream).
+struct KernelTimer {
+    virtual ~KernelTimer() = default;
+    virtual void begin(hipStream_t s) = 0;
+    // name of the kernel just launched and its algorithmic HBM bytes
+    virtual void end(hipStream_t s, const char* name, double bytes) = 0;
+};
+
+struct SweepTuning {
+    int tile_bits;   // TB of the LOW sweep (13 for c64, 12 for c128 by default)
+    int row_bits;    // R of HIGH sweeps
+    int grid;        // workgroups per sweep launch (persistent loop over tiles)
+    KernelTimer* timer = nullptr;
+};
+
+SweepTuning default_tuning(int n, int precision);
+
+// scratch for reductions (partials): bytes needed for a given grid
+size_t partials_bytes(int grid);
+
+// Launch one Trotter step (phase + all rotations) on 'stream'.
+hipError_t launch_step(void* psi, const StepSpec& spec, const SweepTuning& tun, double* partials,
+                       hipStream_t stream);
+
+// Read-only measurement of the state into series_slot (3 doubles).
+hipError_t launch_measure(const void* psi, int n, int precision, uint32_t s0, uint32_t e0,
+                          double* series_slot, double* partials, int grid, hipStream_t stream);
+
+// Phase only (exp(-i dtp E/n)), e.g. the closing half step of Strang.
+hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, hipStream_t stream);
+
+// psi = |s0>
+hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, hipStream_t stream);
+
+// count of s with E(s) < e0 over all 2^n states (exact, integer)
+hipError_t launch_count_below(int n, uint32_t e0, unsigned long long* d_count,
+                              unsigned long long* partials, int grid, hipStream_t stream);
+
+// ---- sampling (inverse CDF over |psi|^2) ----
+// block sums of |psi|^2 (FP64) for blocks of 2^sample_block_bits amplitudes
+constexpr int kSampleBlockBits = 12;
+hipError_t launch_sample(const void* psi, int n, int precision, const double* d_rand, int shots,
+                         int64_t* d_out, double* d_block_sums, double* d_block_prefix,
+                         hipStream_t stream);
+size_t sample_scratch_doubles(int n);
+
+// ---- H.v (matrix-free, diagonalizer.cuh:60-77) ----
+// y = (E/n or E) * x - g * sum_j X_j x ; real FP64 (complex=0) or complex128 (complex=1)
+hipError_t launch_hv(int n, double g, int normalize_by_n, int is_complex, const double* x,
+                     double* y, hipStream_t stream);
+
+// ---- BLAS-1 with deterministic reductions (FP64) ----
+// out[0] = sum_i x_i * y_i over len elements (complex: sum conj(x_i) y_i -> out[0..1])
+hipError_t launch_dot(int64_t len, int is_complex, const double* x, const double* y, double* out,
+                      double* partials, hipStream_t stream);
+// y += a * x  (complex a = (ar, ai) when is_complex)
+hipError_t launch_axpy(int64_t len, int is_complex, double ar, double ai, const double* x,
+                       double* y, hipStream_t stream);
+// x *= a (real)
+hipError_t launch_scal(int64_t len, double a, double* x, hipStream_t stream);
+// y = sum_j c_j * V_j  (m basis vectors of length len, stride len; complex coefficients)
+hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V,
+                          const double* coef, double* y, hipStream_t stream);
+size_t blas_partials_doubles();
+
+}  // namespace bk

@@ -1,0 +1,475 @@
+// bk_linalg.hip -- gfx950 kernels around the Trotter sweeps:
+//  * projective sampling by inverse CDF over |psi|^2 (replaces the cuStateVec
+//    sampler calls of sampler.cuh:464-471, 564-573): FP64 block sums, one
+//    workgroup exclusive scan, one wave per shot for the search;
+//  * matrix-free H.v (diagonalizer.cuh:60-77): y = d*x - g sum_j X_j x with
+//    d = E/N, tiled like the sweeps (a LOW pass with the diagonal and qubits
+//    [0,TB), HIGH passes accumulating the remaining qubits);
+//  * FP64 BLAS-1 with fixed-order reductions for Lanczos/Krylov (replaces the
+//    cuBLAS calls of diagonalizer.cuh:449-538).
+#include "bk_kernels.hpp"
+#include "bk_plan.hpp"
+
+#include <algorithm>
+
+namespace bk {
+
+__device__ __forceinline__ uint32_t labs_energy_u32_l(uint32_t s, int n)
+{
+    uint32_t e = 0;
+#pragma unroll 4
+    for (int k = 1; k < n; ++k) {
+        const uint32_t pairs = (s ^ (s >> k)) & (0xFFFFFFFFu >> (32 - (n - k)));
+        const int c = (n - k) - 2 * static_cast<int>(__popc(pairs));
+        e += static_cast<uint32_t>(c * c);
+    }
+    return e;
+}
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename V>
+__device__ __forceinline__ double prob(const V& x)
+{
+    const double a = static_cast<double>(x.x), b = static_cast<double>(x.y);
+    return fma(a, a, b * b);
+}
+
+// ---------------------------------------------------------------------------
+// Sampling
+// ---------------------------------------------------------------------------
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_block_prob(const V* __restrict__ psi, double* __restrict__ sums)
+{
+    __shared__ double sc[4];
+    const uint64_t beg = static_cast<uint64_t>(blockIdx.x) << kSampleBlockBits;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < (1 << kSampleBlockBits); i += kThreads) a += prob(psi[beg + i]);
+    a = wsum(a);
+    if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = (sc[0] + sc[1]) + (sc[2] + sc[3]);
+}
+
+// exclusive scan of nb block sums into prefix[0..nb] (prefix[nb] = total)
+__global__ __launch_bounds__(kThreads) void k_scan(const double* __restrict__ sums, int nb,
+                                                  double* __restrict__ prefix)
+{
+    __shared__ double seg[kThreads];
+    const int per = (nb + kThreads - 1) / kThreads;
+    const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    double s = 0.0;
+    for (int i = b0; i < b1; ++i) s += sums[i];
+    seg[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int i = 0; i < kThreads; ++i) {
+            const double v = seg[i];
+            seg[i] = run;
+            run += v;
+        }
+        prefix[nb] = run;
+    }
+    __syncthreads();
+    double run = seg[threadIdx.x];
+    for (int i = b0; i < b1; ++i) {
+        prefix[i] = run;
+        run += sums[i];
+    }
+}
+
+// one wave per shot: index = min{ i : cdf_i > r * total }, or dim-1 if none
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_search(const V* __restrict__ psi, int nb,
+                                                    const double* __restrict__ prefix,
+                                                    const double* __restrict__ rnd, int shots,
+                                                    int64_t* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int shot = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (shot >= shots) return;
+    const double total = prefix[nb];
+    const double target = rnd[shot] * total;
+    // smallest b with prefix[b+1] > target
+    int lo = 0, hi = nb;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid + 1] > target) hi = mid;
+        else lo = mid + 1;
+    }
+    const int64_t dim = static_cast<int64_t>(nb) << kSampleBlockBits;
+    int64_t found = -1;
+    for (int b = lo; b < nb && found < 0; ++b) {
+        double run = prefix[b];
+        const int64_t beg = static_cast<int64_t>(b) << kSampleBlockBits;
+        for (int c = 0; c < (1 << kSampleBlockBits) && found < 0; c += 64) {
+            double v = prob(psi[beg + c + lane]);
+            // inclusive wave scan
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const double u = __shfl_up(v, off, 64);
+                if (lane >= off) v += u;
+            }
+            const double cum = run + v;
+            const unsigned long long hit = __ballot(cum > target);
+            if (hit) found = beg + c + __ffsll(static_cast<long long>(hit)) - 1;
+            run = __shfl(cum, 63, 64);
+        }
+    }
+    if (found < 0) found = dim - 1;
+    if (lane == 0) out[shot] = found;
+}
+
+size_t sample_scratch_doubles(int n)
+{
+    const int bits = std::max(n - kSampleBlockBits, 0);
+    return 2 * ((size_t(1) << bits) + 1);
+}
+
+template <typename V>
+__global__ void k_search_small(const V* __restrict__ psi, int64_t dim, const double* __restrict__ rnd,
+                               int shots, int64_t* __restrict__ out)
+{
+    // tiny states (< one block): sequential FP64 CDF per shot
+    const int shot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (shot >= shots) return;
+    double total = 0.0;
+    for (int64_t i = 0; i < dim; ++i) total += prob(psi[i]);
+    const double target = rnd[shot] * total;
+    double run = 0.0;
+    int64_t idx = dim - 1;
+    for (int64_t i = 0; i < dim; ++i) {
+        run += prob(psi[i]);
+        if (run > target) {
+            idx = i;
+            break;
+        }
+    }
+    out[shot] = idx;
+}
+
+template <typename V>
+static hipError_t sample_impl(const V* psi, int n, const double* rnd, int shots, int64_t* out, double* sums,
+                              double* prefix, hipStream_t st)
+{
+    if (shots <= 0) return hipSuccess;
+    if (n < kSampleBlockBits) {
+        hipLaunchKernelGGL(k_search_small<V>, dim3((shots + 63) / 64), dim3(64), 0, st, psi, int64_t(1) << n,
+                           rnd, shots, out);
+        return hipGetLastError();
+    }
+    const int nb = 1 << (n - kSampleBlockBits);
+    hipLaunchKernelGGL(k_block_prob<V>, dim3(nb), dim3(kThreads), 0, st, psi, sums);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kThreads), 0, st, sums, nb, prefix);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int wpb = kThreads / 64;
+    hipLaunchKernelGGL(k_search<V>, dim3((shots + wpb - 1) / wpb), dim3(kThreads), 0, st, psi, nb, prefix, rnd,
+                       shots, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample(const void* psi, int n, int precision, const double* d_rand, int shots, int64_t* d_out,
+                         double* d_block_sums, double* d_block_prefix, hipStream_t stream)
+{
+    if (precision == kC64)
+        return sample_impl(static_cast<const float2*>(psi), n, d_rand, shots, d_out, d_block_sums,
+                           d_block_prefix, stream);
+    if (precision == kC128)
+        return sample_impl(static_cast<const double2*>(psi), n, d_rand, shots, d_out, d_block_sums,
+                           d_block_prefix, stream);
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// H.v
+// ---------------------------------------------------------------------------
+struct Dbl1 {
+    double v;
+};
+
+template <typename T>
+__device__ __forceinline__ T tz()
+{
+    return T{};
+}
+__device__ __forceinline__ void axpy_(double& y, double a, double x) { y = fma(a, x, y); }
+__device__ __forceinline__ void axpy_(double2& y, double a, const double2& x)
+{
+    y.x = fma(a, x.x, y.x);
+    y.y = fma(a, x.y, y.y);
+}
+__device__ __forceinline__ double scl(double a, double x) { return a * x; }
+__device__ __forceinline__ double2 scl(double a, const double2& x) { return make_double2(a * x.x, a * x.y); }
+
+template <typename T>
+struct HvCfg;
+template <>
+struct HvCfg<double> {
+    static constexpr int TB = 13;
+};
+template <>
+struct HvCfg<double2> {
+    static constexpr int TB = 12;
+};
+
+// LOW pass: y = d*x - g * sum_{j<TB} x[s ^ 2^j] on contiguous tiles
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_hv_low(const T* __restrict__ x, T* __restrict__ y, uint64_t ntiles,
+                                                    int n, double g, double dscale)
+{
+    constexpr int TB = HvCfg<T>::TB;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t base = tile << TB;
+        __syncthreads();
+        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) lds[i] = x[base + i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) {
+            const uint32_t s = static_cast<uint32_t>(base + i);
+            T acc = scl(static_cast<double>(labs_energy_u32_l(s, n)) * dscale, lds[i]);
+#pragma unroll
+            for (int j = 0; j < TB; ++j) axpy_(acc, -g, lds[i ^ (1 << j)]);
+            y[base + i] = acc;
+        }
+    }
+}
+
+// HIGH pass: y += -g * sum_{j in [h, h+A)} x[s ^ 2^j]; tile = 2^R rows x 2^A
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_hv_high(const T* __restrict__ x, T* __restrict__ y, uint64_t ntiles,
+                                                     int R, int h, double g)
+{
+    constexpr int TB = HvCfg<T>::TB;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    const int A = TB - R;
+    const int lo_bits = h - R;
+    const uint32_t rowmask = (1u << R) - 1u;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t lo = tile & ((1ull << lo_bits) - 1ull);
+        const uint64_t hi = tile >> lo_bits;
+        const uint64_t base = (lo << R) | (hi << (h + A));
+        __syncthreads();
+        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) {
+            const uint64_t gi = base + (i & rowmask) + (static_cast<uint64_t>(i >> R) << h);
+            lds[i] = x[gi];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) {
+            const uint64_t gi = base + (i & rowmask) + (static_cast<uint64_t>(i >> R) << h);
+            T acc = y[gi];
+            for (int j = 0; j < A; ++j) axpy_(acc, -g, lds[i ^ (1 << (R + j))]);
+            y[gi] = acc;
+        }
+    }
+}
+
+// small n: one row per thread, as diagonalizer.cuh:60-77
+template <typename T>
+__global__ void k_hv_rows(const T* __restrict__ x, T* __restrict__ y, int n, double g, double dscale)
+{
+    const uint64_t dim = 1ull << n;
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= dim) return;
+    T acc = scl(static_cast<double>(labs_energy_u32_l(static_cast<uint32_t>(i), n)) * dscale, x[i]);
+    for (int j = 0; j < n; ++j) axpy_(acc, -g, x[i ^ (1ull << j)]);
+    y[i] = acc;
+}
+
+template <typename T>
+static hipError_t hv_impl(int n, double g, double dscale, const T* x, T* y, hipStream_t st)
+{
+    constexpr int TB = HvCfg<T>::TB;
+    const uint64_t dim = 1ull << n;
+    hipError_t e;
+    if (n < TB) {
+        hipLaunchKernelGGL(k_hv_rows<T>, dim3(static_cast<unsigned>((dim + 255) / 256)), dim3(256), 0, st, x, y, n,
+                           g, dscale);
+        return hipGetLastError();
+    }
+    const size_t lds = sizeof(T) << TB;
+    const uint64_t ntiles = dim >> TB;
+    const int grid = static_cast<int>(std::min<uint64_t>(ntiles, 2048));
+    hipLaunchKernelGGL(k_hv_low<T>, dim3(grid), dim3(kThreads), lds, st, x, y, ntiles, n, g, dscale);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int rmin = 4;
+    const int amax = TB - rmin;
+    const int remaining = n - TB;
+    if (remaining <= 0) return hipSuccess;
+    const int nh = (remaining + amax - 1) / amax;
+    int h = TB;
+    for (int i = 0; i < nh; ++i) {
+        const int left = n - h;
+        const int A = (left + (nh - i) - 1) / (nh - i);
+        const int R = TB - A;
+        hipLaunchKernelGGL(k_hv_high<T>, dim3(grid), dim3(kThreads), lds, st, x, y, ntiles, R, h, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        h += A;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_hv(int n, double g, int normalize_by_n, int is_complex, const double* x, double* y,
+                     hipStream_t stream)
+{
+    if (n < 1 || n > 32) return hipErrorInvalidValue;
+    const double dscale = normalize_by_n ? 1.0 / static_cast<double>(n) : 1.0;
+    if (is_complex)
+        return hv_impl(n, g, dscale, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), stream);
+    return hv_impl(n, g, dscale, x, y, stream);
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1 (FP64, fixed-order reductions)
+// ---------------------------------------------------------------------------
+constexpr int kBlasGrid = 1024;
+size_t blas_partials_doubles() { return 2 * kBlasGrid + 8; }
+
+template <bool CPLX>
+__global__ __launch_bounds__(kThreads) void k_dot(int64_t len, const double* __restrict__ x,
+                                                 const double* __restrict__ y, int64_t chunk,
+                                                 double* __restrict__ partials)
+{
+    __shared__ double sc[8];
+    double a = 0.0, b = 0.0;
+    const int64_t beg = blockIdx.x * chunk;
+    const int64_t end = min(len, beg + chunk);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        if (CPLX) {
+            const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+            a = fma(xr, yr, fma(xi, yi, a));
+            b = fma(xr, yi, fma(-xi, yr, b));
+        } else {
+            a = fma(x[i], y[i], a);
+        }
+    }
+    a = wsum(a);
+    if (CPLX) b = wsum(b);
+    if ((threadIdx.x & 63) == 0) {
+        sc[2 * (threadIdx.x >> 6)] = a;
+        sc[2 * (threadIdx.x >> 6) + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = (sc[0] + sc[2]) + (sc[4] + sc[6]);
+        partials[2 * blockIdx.x + 1] = (sc[1] + sc[3]) + (sc[5] + sc[7]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_dot_finish(const double* __restrict__ partials, int count,
+                                                        double* __restrict__ out)
+{
+    __shared__ double sc[8];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    a = wsum(a);
+    b = wsum(b);
+    if ((threadIdx.x & 63) == 0) {
+        sc[2 * (threadIdx.x >> 6)] = a;
+        sc[2 * (threadIdx.x >> 6) + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = (sc[0] + sc[2]) + (sc[4] + sc[6]);
+        out[1] = (sc[1] + sc[3]) + (sc[5] + sc[7]);
+    }
+}
+
+hipError_t launch_dot(int64_t len, int is_complex, const double* x, const double* y, double* out, double* partials,
+                      hipStream_t st)
+{
+    int64_t chunk = 4096;
+    while ((len + chunk - 1) / chunk > kBlasGrid) chunk <<= 1;
+    const int grid = static_cast<int>(std::max<int64_t>(1, (len + chunk - 1) / chunk));
+    if (is_complex)
+        hipLaunchKernelGGL(k_dot<true>, dim3(grid), dim3(kThreads), 0, st, len, x, y, chunk, partials);
+    else
+        hipLaunchKernelGGL(k_dot<false>, dim3(grid), dim3(kThreads), 0, st, len, x, y, chunk, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(kThreads), 0, st, partials, grid, out);
+    return hipGetLastError();
+}
+
+__global__ void k_axpy(int64_t len, int cplx, double ar, double ai, const double* __restrict__ x,
+                       double* __restrict__ y)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < len;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (cplx) {
+            const double xr = x[2 * i], xi = x[2 * i + 1];
+            y[2 * i] = fma(ar, xr, fma(-ai, xi, y[2 * i]));
+            y[2 * i + 1] = fma(ar, xi, fma(ai, xr, y[2 * i + 1]));
+        } else {
+            y[i] = fma(ar, x[i], y[i]);
+        }
+    }
+}
+
+hipError_t launch_axpy(int64_t len, int is_complex, double ar, double ai, const double* x, double* y,
+                       hipStream_t st)
+{
+    const int grid = static_cast<int>(std::min<int64_t>((len + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_axpy, dim3(std::max(grid, 1)), dim3(256), 0, st, len, is_complex, ar, ai, x, y);
+    return hipGetLastError();
+}
+
+__global__ void k_scal(int64_t len, double a, double* __restrict__ x)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < len;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        x[i] *= a;
+}
+
+hipError_t launch_scal(int64_t len, double a, double* x, hipStream_t st)
+{
+    const int grid = static_cast<int>(std::min<int64_t>((len + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_scal, dim3(std::max(grid, 1)), dim3(256), 0, st, len, a, x);
+    return hipGetLastError();
+}
+
+// y = sum_j coef_j * V_j ; coef complex (re, im) pairs when cplx
+__global__ void k_combine(int64_t len, int cplx, int m, const double* __restrict__ V, const double* __restrict__ coef,
+                          double* __restrict__ y)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < len;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (cplx) {
+            double ar = 0.0, ai = 0.0;
+            for (int j = 0; j < m; ++j) {
+                const double cr = coef[2 * j], ci = coef[2 * j + 1];
+                const double vr = V[(2 * len) * j + 2 * i], vi = V[(2 * len) * j + 2 * i + 1];
+                ar = fma(cr, vr, fma(-ci, vi, ar));
+                ai = fma(cr, vi, fma(ci, vr, ai));
+            }
+            y[2 * i] = ar;
+            y[2 * i + 1] = ai;
+        } else {
+            double a = 0.0;
+            for (int j = 0; j < m; ++j) a = fma(coef[j], V[len * j + i], a);
+            y[i] = a;
+        }
+    }
+}
+
+hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V, const double* coef, double* y,
+                          hipStream_t st)
+{
+    const int grid = static_cast<int>(std::min<int64_t>((len + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_combine, dim3(std::max(grid, 1)), dim3(256), 0, st, len, is_complex, m, V, coef, y);
+    return hipGetLastError();
+}
+
+}  // namespace bk

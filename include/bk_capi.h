@@ -1,0 +1,224 @@
+/*
+ * bk_capi.h -- C ABI of the MI355X-native Trotter engine (libbk.so).
+ *
+ * Drop-in boundary for the reference's src/sampling and src/subspace paths.
+ * Plain C: POD structs, pointers and sizes, int error codes (0 = success,
+ * negative = error, message in bk_last_error()); no exceptions or C++/torch
+ * types cross the ABI.  A context owns one device, one HIP stream and one
+ * statevector; calls on one context are not thread-safe (the reference's
+ * CuStateVecSampler is not either, sampler.cuh:318-338).
+ *
+ * Reference interfaces replaced (file:line relative to the reference repo):
+ *   ISampler::sample / create_sampler ........ sampler.cuh:303-310, 737-743, 412-707
+ *   precompute / phase / mixer / reductions .. sampler.cuh:54-227, 604-617
+ *   custatevecSampler{Create,Preprocess,Sample} sampler.cuh:464-471, 564-573
+ *   matrix-free H.v MatVecFunc ............... diagonalizer.cuh:60-77, 400-406, 420-424
+ *   run_gpu_lanczos_internal ................. diagonalizer.cuh:420-554
+ *   LabsEvaluator::calcEnergy / compute_d_min  labs_energy.hpp:121-132, 175-207
+ *   ClassicalOptimizer random_state + steepest_descent classical.cuh:602-713
+ */
+#ifndef BK_CAPI_H
+#define BK_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BK_OK 0
+#define BK_E_INVALID (-1)   /* bad argument (n out of range, NaN field, null pointer, ...) */
+#define BK_E_HIP (-2)       /* HIP runtime error (message names the call) */
+#define BK_E_NOMEM (-3)     /* device or host allocation failed */
+#define BK_E_NOSTATE (-4)   /* the context holds no statevector yet */
+#define BK_E_NODEVICE (-5)  /* no HIP device */
+
+#define BK_C64 1  /* complex64: float2 amplitudes, FP32 butterflies */
+#define BK_C128 2 /* complex128: double2 amplitudes, FP64 throughout */
+
+#define BK_LIE 1    /* first order, phase first: U = X(g dt) P(dt)  (sampler.cuh:604-617) */
+#define BK_STRANG 2 /* second order: U = P(dt/2) X(g dt) P(dt/2) */
+
+#define BK_MAX_N 32
+
+typedef struct bk_ctx bk_ctx;
+
+/* Last error message of the calling thread ("" when none). */
+const char* bk_last_error(void);
+int bk_version(void);
+/* Number of HIP devices (0 when none), without creating a context. */
+int bk_device_count(void);
+
+/* One context per device (replaces custatevecCreate/Destroy, sampler.cuh:448-449, 703). */
+int bk_ctx_create(int device, bk_ctx** out);
+void bk_ctx_destroy(bk_ctx* ctx);
+
+/* ---- L0 math core (host, exact integers) ---- */
+/* labs_energy.hpp:121-132 */
+int64_t bk_energy(int n, uint64_t s);
+/* labs_energy.hpp:175-207 (target < 0 -> E(s0); returns the reference's not-found value) */
+int bk_d_min(int n, uint64_t s0, int64_t target, int max_d);
+/* labs_energy.hpp:156-168 (-1 outside 3..32) */
+int64_t bk_known_optimal_energy(int n);
+/* classical.cuh:602-604 (first draw of mt19937_64(seed)) + :625-713 (steepest descent) */
+int bk_make_trap(int n, uint64_t seed, uint64_t* start, uint64_t* s0, int64_t* e0);
+
+/* ---- statevector held by the context ---- */
+/* allocate 2^n amplitudes in 'precision' and set |start_bits> (sampler.cuh:428-438) */
+int bk_state_init(bk_ctx* ctx, int n, int precision, uint64_t start_bits);
+/* copy a host statevector in (interleaved re,im; float for C64, double for C128) */
+int bk_state_upload(bk_ctx* ctx, int n, int precision, const void* host_amps);
+/* copy the statevector out (any pending closing Strang half-phase is applied first) */
+int bk_state_download(bk_ctx* ctx, void* host_amps);
+/* device pointer and stream of the held state (for zero-copy interop) */
+int bk_state_device(bk_ctx* ctx, void** d_amps, void** hip_stream, int* n, int* precision);
+
+/* ---- evolution (sampler.cuh:509-619 without the sampling) ---- */
+typedef struct {
+    int n;              /* 1..32 */
+    int precision;      /* BK_C64 | BK_C128 */
+    int order;          /* BK_LIE | BK_STRANG */
+    double dt;          /* Trotter step */
+    int steps;          /* number of steps */
+    double g;           /* constant field (used when g_steps == NULL) */
+    const double* g_steps; /* optional per-step field, length 'steps' (host memory) */
+    uint64_t start_bits;   /* |s0>, also the reference for <d_H> */
+    int64_t e0;            /* P_< threshold; < 0 -> E(start_bits) */
+    int init_from_start;   /* 1: state <- |start_bits>; 0: continue from the held state */
+    int measure_every;     /* record observables every k steps (0 -> only after the last) */
+} bk_evolve_params;
+
+/* Observables recorded by bk_evolve: slot i holds the state after i*measure_every
+ * steps (the last slot the final state).  Caller-owned host arrays of 'capacity'
+ * entries; 'count' is set to the number written. */
+typedef struct {
+    double* p_less;  /* sum_{E(s)<E0} |psi_s|^2          (sampler.cuh:149-195) */
+    double* dh;      /* sum popcount(s ^ s0) |psi_s|^2 */
+    double* norm;    /* sum |psi_s|^2 (drift diagnostic) */
+    int64_t capacity;
+    int64_t count;
+    int64_t e0;      /* threshold used */
+    double p_blind;  /* #{E < E0} / 2^n  (sampler.cuh:198-227, 383-399); -1 when
+                        init_from_start == 0 (not recomputed for a continuation) */
+} bk_series;
+
+int bk_evolve(bk_ctx* ctx, const bk_evolve_params* p, bk_series* out);
+
+/* P_<, <d_H>, norm of the held state (out[3]) */
+int bk_observables(bk_ctx* ctx, uint64_t s0, int64_t e0, double* out3);
+/* exact #{ s : E(s) < e0 } over 2^n states, on the device */
+int bk_count_below(bk_ctx* ctx, int n, int64_t e0, uint64_t* count);
+
+/* ---- projective sampling (custatevecSamplerPreprocess + Sample with
+ * CUSTATEVEC_SAMPLER_OUTPUT_RANDNUM_ORDER, sampler.cuh:564-573):
+ * out_idx[i] = min{ s : cdf(s) > randnums[i] * cdf(2^n - 1) } on the held state. */
+int bk_sample(bk_ctx* ctx, const double* randnums, int64_t shots, int64_t* out_idx);
+
+/* ---- matrix-free H.v (diagonalizer.cuh:60-77; the MatVecFunc of :400-406) ----
+ * y = (E/N or E) * x - g * sum_j X_j x on DEVICE pointers; is_complex selects
+ * complex128 (interleaved) over FP64 real.  stream may be NULL (context stream). */
+int bk_hv(bk_ctx* ctx, int n, double g, int normalize_by_n, int is_complex, const double* d_x,
+          double* d_y, void* hip_stream);
+
+/* ---- Lanczos (diagonalizer.cuh:420-554: v0 from mt19937_64(42) uniform(-0.5,0.5),
+ * no reorthogonalisation, breakdown tol, Ritz vectors) on the full 2^n space. ----
+ * evals: k_pairs ascending; evecs (host, k_pairs*2^n, may be NULL). Returns the
+ * number of pairs (>= 0) or an error code. */
+int bk_lanczos(bk_ctx* ctx, int n, double g, int normalize_by_n, int max_krylov_dim, double tol,
+               int k_pairs, double* evals, double* evecs);
+
+/* ---- Krylov propagator: held complex128 state <- exp(-i H tau) state, m-step
+ * Lanczos with full reorthogonalisation, restarted every tau_restart. ---- */
+int bk_krylov_propagate(bk_ctx* ctx, double g, int normalize_by_n, double tau, double tau_restart,
+                        int m);
+
+/* ---- ISampler::sample as one call (sampler.cuh:412-707) ---- */
+typedef struct {            /* SamplerConfig, sampler.cuh:235-260 */
+    int n;
+    double g;
+    double dt;
+    double t_max;
+    int num_sample_points;
+    int shots_per_point;
+    uint64_t seed;
+    int num_sampling_seeds;
+    uint64_t seed_stride;
+    double t_start;
+    double dt_sample;
+    int track_p_less;
+    int verbose;
+    int record_trajectory;
+    double dt_trajectory;
+    /* engine extensions (defaults: BK_C128, BK_LIE reproduce the reference) */
+    int precision;
+    int order;
+} bk_sampler_config;
+
+/* the values of SamplerConfig's member initialisers (sampler.cuh:235-260) */
+void bk_sampler_config_defaults(bk_sampler_config* cfg);
+
+typedef struct {            /* SeedSamplingStats, sampler.cuh:262-270 */
+    uint64_t seed;
+    uint64_t total_shots_collected;
+    uint64_t improved_shots;
+    double success_rate;
+    double per_eval_rate;
+    double mean_hamming_flips;
+    int64_t best_energy;
+} bk_seed_stats;
+
+typedef struct {            /* SamplingResult, sampler.cuh:272-297; library-owned arrays */
+    uint64_t* unique_basis;
+    int64_t unique_basis_len;
+    uint64_t total_shots_collected;
+    uint64_t improved_shots;
+    double success_rate;
+    double per_eval_rate;
+    uint64_t total_evals;
+    double mean_hamming_flips;
+    bk_seed_stats* seed_stats;
+    int64_t seed_stats_len;
+    double* time_points;
+    double* p_less_history;
+    double* dh_history;
+    int64_t history_len;
+    double window_avg_dh;
+    double initial_rise_rate_avg;
+    double initial_rise_rate_max;
+    int d_min;
+    int64_t start_energy;
+    double p_blind;
+    double peak_p_less;
+    double peak_time;
+    /* engine extension: sampled indices in draw order (point-major, seed, shot) */
+    int64_t* samples;
+    int64_t samples_len;
+} bk_sampling_result;
+
+/* Throws nothing; n > BK_MAX_N returns BK_E_INVALID (the reference throws for
+ * n > 30, sampler.cuh:416-418, this engine runs to 32). */
+int bk_sampler_run(bk_ctx* ctx, const bk_sampler_config* cfg, uint64_t start_bits,
+                   bk_sampling_result** out);
+void bk_result_free(bk_sampling_result* r);
+
+/* ---- instrumentation (no reference counterpart; the reference only wall-times
+ * the whole sample() call, connector.cuh:627, 832-833) ----
+ * When enabled, every sweep launch of bk_evolve / bk_sampler_run is bracketed
+ * by HIP events on the context stream.  bk_timing_read synchronises, returns
+ * one row per kernel name (launch count, summed device ms, algorithmic HBM
+ * bytes per launch) and clears the accumulated rows. */
+typedef struct {
+    char name[48];
+    int64_t launches;
+    double total_ms;
+    double bytes_per_launch;
+} bk_kernel_timing;
+
+int bk_timing_enable(bk_ctx* ctx, int enable);
+int bk_timing_read(bk_ctx* ctx, bk_kernel_timing* out, int cap, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

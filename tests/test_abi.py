@@ -1,0 +1,89 @@
+"""CPU tests of the C ABI surface: libbk.so loads, exports every function that
+include/bk_capi.h declares, reports the absence of a device cleanly, and its
+host-side integer math (energy, d_min, trap construction) agrees with the
+oracle.  No GPU compute is called here."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import random
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def _declared_functions():
+    src = open(os.path.join(ROOT, "include", "bk_capi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b(bk_[a-z0-9_]+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_library_exports_all_declared_symbols(bk):
+    L = bk.lib()
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(L, name), name
+    # the Python mirror binds exactly the declared set
+    assert sorted(bk.exported_symbols()) == declared
+
+
+def test_no_device_path(bk):
+    if bk.lib().bk_device_count() > 0:
+        pytest.skip("a device is present")
+    h = C.c_void_p()
+    rc = bk.lib().bk_ctx_create(0, C.byref(h))
+    assert rc == -5 and not h.value
+    assert "no HIP device" in bk.lib().bk_last_error().decode()
+    with pytest.raises(bk.BKError):
+        bk.Engine(0)
+
+
+def test_host_math_matches_oracle(bk, orc):
+    rng = random.Random(17)
+    for _ in range(2000):
+        n = rng.randint(2, 32)
+        s = rng.getrandbits(n)
+        assert bk.energy(n, s) == orc.energy(n, s)
+    for n in range(0, 34):
+        assert bk.known_optimal_energy(n) == orc.known_optimal_energy(n)
+    for _ in range(200):
+        n = rng.randint(3, 18)
+        s = rng.getrandbits(n)
+        assert bk.d_min(n, s) == orc.d_min(n, s)
+        md = rng.randint(1, n)
+        assert bk.d_min(n, s, -1, md) == orc.d_min(n, s, -1, md)
+    for seed in range(60):
+        n = rng.choice([12, 16, 20, 24, 30, 32])
+        assert bk.make_trap(n, seed) == orc.make_trap(n, seed)
+
+
+def test_trap_with_dmin_filter(bk, orc):
+    """Config-1 trap rule: iterate seeds until exact d_min >= 6 (N=24)."""
+    seed, start, s0, e0, dm = bk.make_trap_min_dmin(24, 1, 6)
+    assert dm >= 6 and orc.d_min(24, s0, e0) == dm
+    assert orc.make_trap(24, seed) == (start, s0, e0)
+
+
+def test_sampler_config_defaults(bk):
+    cfg = bk._SamplerConfigC()
+    bk.lib().bk_sampler_config_defaults(C.byref(cfg))
+    py = bk.SamplerConfig()
+    for f in ("n", "g", "dt", "t_max", "num_sample_points", "shots_per_point", "seed",
+              "num_sampling_seeds", "seed_stride", "t_start", "dt_sample", "dt_trajectory"):
+        assert getattr(cfg, f) == getattr(py, f), f
+    assert bool(cfg.track_p_less) is py.track_p_less
+    assert bool(cfg.record_trajectory) is py.record_trajectory
+    # sampler.cuh:235-260 initialisers
+    assert (py.n, py.g, py.dt, py.t_max, py.num_sample_points, py.shots_per_point) == \
+        (16, 0.5, 0.05, 60.0, 20, 100)
+    assert (py.seed, py.num_sampling_seeds, py.seed_stride) == (42, 3, 10007)
+
+
+def test_factory_above_max_n_raises(bk):
+    s = bk.create_sampler(33)
+    with pytest.raises(RuntimeError):
+        s.sample(0, bk.SamplerConfig(n=33))

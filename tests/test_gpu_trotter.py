@@ -1,0 +1,138 @@
+"""GPU parity of the Trotter hot path (LOW/HIGH sweeps, fused phase and
+reductions) against the FP64 oracle and the reference's golden trajectories.
+
+Tolerances: complex128 runs match the FP64 oracle to 1e-12 (P_<, norm) /
+1e-10 (<d_H>, which is O(N)); complex64 runs match to 1e-5 in P_< (the stated
+target), 1e-4 in <d_H>, and 1 - |<psi64|psi128>| <= 1e-5."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C64, C128, LIE, STRANG = 1, 2, 1, 2
+
+CASES = [
+    # n, precision, order, g, dt, steps, seed
+    (4, C128, LIE, 0.5, 0.05, 40, 1),
+    (10, C128, LIE, 0.35, 0.05, 100, 2),
+    (12, C128, STRANG, 0.35, 0.02, 100, 3),   # c128 LOW sweep only
+    (13, C64, LIE, 0.35, 0.05, 100, 4),       # c64 LOW sweep only
+    (16, C128, LIE, 0.3, 0.05, 80, 5),        # LOW + 1 HIGH
+    (16, C64, STRANG, 0.3, 0.02, 80, 6),
+    (20, C128, STRANG, 0.35, 0.02, 40, 7),
+    (21, C64, LIE, 0.45, 0.05, 40, 8),
+    (22, C128, LIE, 0.35, 0.05, 20, 9),       # LOW + 2 HIGH (c128)
+    (23, C64, STRANG, 0.35, 0.02, 20, 10),    # LOW + 2 HIGH (c64)
+]
+
+
+def _tol(prec):
+    return (1e-12, 1e-10) if prec == C128 else (1e-5, 1e-4)
+
+
+@pytest.mark.parametrize("n,prec,order,g,dt,steps,seed", CASES)
+def test_trotter_parity_from_trap(engine, orc, bk, n, prec, order, g, dt, steps, seed):
+    print(f"case n={n} prec={prec} order={order} steps={steps}", flush=True)
+    _, s0, e0 = orc.make_trap(n, seed)
+    ref = orc.trotter(n, s0, g, dt, steps, order=order, want_state=True)
+    got = engine.evolve(n, s0, g, dt, steps, order=order, precision=prec)
+    tp, td = _tol(prec)
+    assert got["e0"] == e0
+    assert np.max(np.abs(got["p_less"] - ref["p_less"])) <= tp
+    assert np.max(np.abs(got["dh"] - ref["dh"])) <= td
+    assert np.max(np.abs(got["norm"] - ref["norm"])) <= (1e-12 if prec == C128 else 1e-5)
+    psi = engine.download().astype(np.complex128)
+    if prec == C128:
+        assert np.max(np.abs(psi - ref["psi"])) <= 1e-11
+    else:
+        ov = abs(np.vdot(ref["psi"], psi))
+        assert 1.0 - ov <= 1e-5
+    assert got["p_blind"] == pytest.approx(orc.count_below(n, e0) / 2.0 ** n, rel=0, abs=0)
+
+
+@pytest.mark.parametrize("n,prec", [(11, C128), (14, C128), (15, C64), (19, C64), (24, C64)])
+def test_trotter_parity_generic_state(engine, orc, n, prec):
+    """Random input state: exercises every phase/mixer entry, not just a basis state."""
+    print(f"generic n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(n)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    if prec == C64:
+        psi = psi.astype(np.complex64).astype(np.complex128)
+    s0, e0, g, dt, steps = 5, orc.energy(n, 5), 0.41, 0.03, 12
+    engine.upload(psi.astype(np.complex64 if prec == C64 else np.complex128), prec)
+    got = engine.evolve(n, s0, g, dt, steps, order=STRANG, precision=prec, init=False)
+    ref = orc.trotter(n, s0, g, dt, steps, order=2, psi=psi, want_state=True)
+    tp, td = _tol(prec)
+    assert np.max(np.abs(got["p_less"] - ref["p_less"])) <= tp
+    assert np.max(np.abs(got["dh"] - ref["dh"])) <= td
+    out = engine.download().astype(np.complex128)
+    tol = 1e-11 if prec == C128 else 3e-6
+    assert np.max(np.abs(out - ref["psi"])) <= tol
+
+
+def test_field_schedule_and_large_angle(engine, orc):
+    """Piecewise g(t) and |cos(g dt)| < 0.5 (full cos/sin butterfly form)."""
+    n, steps = 17, 24
+    _, s0, e0 = orc.make_trap(n, 12)
+    g_steps = np.repeat([0.3, 0.5, 1.4, 0.2], steps // 4)
+    for prec in (C128, C64):
+        got = engine.evolve(n, s0, 0.0, 1.0, steps, order=LIE, precision=prec, g_steps=g_steps)
+        ref = orc.trotter(n, s0, 0.0, 1.0, steps, order=1, g_steps=g_steps)
+        tp, td = _tol(prec)
+        assert np.max(np.abs(got["p_less"] - ref["p_less"])) <= tp * 10
+        assert np.max(np.abs(got["dh"] - ref["dh"])) <= td * 10
+
+
+def test_measure_every_and_continue(engine, orc):
+    n, g, dt = 18, 0.35, 0.02
+    _, s0, e0 = orc.make_trap(n, 13)
+    ref = orc.trotter(n, s0, g, dt, 30, order=2)
+    a = engine.evolve(n, s0, g, dt, 10, order=STRANG, precision=C128, measure_every=5)
+    assert len(a["p_less"]) == 3
+    assert np.allclose(a["p_less"], ref["p_less"][[0, 5, 10]], atol=1e-13, rtol=0)
+    b = engine.evolve(n, s0, g, dt, 20, order=STRANG, precision=C128, init=False,
+                      measure_every=0)
+    assert len(b["p_less"]) == 1
+    assert abs(b["p_less"][0] - ref["p_less"][30]) <= 1e-12
+    assert abs(b["dh"][0] - ref["dh"][30]) <= 1e-10
+
+
+def _golden(golden, n):
+    return [t for t in golden("trajectories.json") if t["n"] == n]
+
+
+@pytest.mark.parametrize("n", [16, 20, 24, 26, 28, 30])
+def test_reference_golden_trajectories_c128(engine, golden, n):
+    """data_processed/probability_dynamics_n{N}.csv (reference GPU runs, c128 Lie,
+    dt=0.05, 1200 steps) reproduced to their printed precision."""
+    for tr in _golden(golden, n):
+        print(f"golden n={n} rank={tr['trap_rank']} g={tr['g']}", flush=True)
+        s0 = int(tr["trap_bits"], 16)
+        got = engine.evolve(n, s0, tr["g"], tr["dt"], tr["steps_total"], order=LIE,
+                            precision=C128)
+        idx = np.array(tr["step"])
+        pl = np.array([float(x) for x in tr["p_less"]])
+        dh = np.array([float(x) for x in tr["dh"]])
+        big = pl > 1e-25
+        assert np.all(np.abs(got["p_less"][idx][big] - pl[big]) <= 6e-7 * pl[big] + 1e-30)
+        assert np.max(np.abs(got["dh"][idx] - dh)) <= 5.1e-5
+        assert abs(got["p_blind"] - float(tr["p_blind"])) <= 5e-7 * float(tr["p_blind"])
+        assert got["e0"] == tr["trap_energy"]
+
+
+def test_c64_against_c128_n24(engine, bk):
+    """Config-1 shape: N=24 trap with d_min >= 6, Strang dt=0.02; c64 vs c128."""
+    seed, start, s0, e0, dm = bk.make_trap_min_dmin(24, 1, 6)
+    steps = 200
+    a = engine.evolve(24, s0, 0.35, 0.02, steps, order=STRANG, precision=C128, measure_every=10)
+    psi128 = engine.download()
+    b = engine.evolve(24, s0, 0.35, 0.02, steps, order=STRANG, precision=C64, measure_every=10)
+    psi64 = engine.download().astype(np.complex128)
+    assert np.max(np.abs(a["p_less"] - b["p_less"])) <= 1e-5
+    assert np.max(np.abs(a["dh"] - b["dh"])) <= 1e-4
+    assert 1.0 - abs(np.vdot(psi128, psi64)) <= 1e-5
