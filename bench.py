@@ -64,6 +64,8 @@ def main() -> int:
     ap.add_argument("--dt", type=float, default=0.02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=30)
+    ap.add_argument("--energy", choices=["cache16", "onthefly"], default="cache16",
+                    help="how the first sweep obtains E(s)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -77,6 +79,8 @@ def main() -> int:
     n = args.n
     start, s0, e0 = bk.make_trap(n, args.seed)
     eng = bk.Engine(0)
+    eng.set_energy_mode(bk.BK_ENERGY_CACHE16 if args.energy == "cache16"
+                        else bk.BK_ENERGY_ONTHEFLY)
 
     # warmup (fresh start from the trap), then the timed continuation
     print(f"[bench] N={n} {args.precision} warmup={args.warmup} steps={args.steps}",
@@ -131,7 +135,7 @@ def main() -> int:
         "config": {
             "workload": f"N={n} {args.precision} Strang dt={args.dt} g={args.g} trotter step, "
                         f"P_< and <d_H> (FP64) fused every step",
-            "n": n, "trap_seed": args.seed, "trap_bits": hex(s0), "E0": e0,
+            "n": n, "energy": args.energy, "trap_seed": args.seed, "trap_bits": hex(s0), "E0": e0,
             "p_less_final": float(res["p_less"][-1]), "norm_final": float(res["norm"][-1]),
             "step_hbm_bytes": step_bytes,
             "step_hbm_frac": step_bytes / (step_dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,

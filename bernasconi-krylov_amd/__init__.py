@@ -36,6 +36,8 @@ BK_C128 = 2
 BK_LIE = 1
 BK_STRANG = 2
 BK_MAX_N = 32
+BK_ENERGY_ONTHEFLY = 0
+BK_ENERGY_CACHE16 = 1
 
 
 class BKError(RuntimeError):
@@ -130,6 +132,8 @@ _SIGS = {
     "bk_sampler_run": (C.c_int, [C.c_void_p, C.POINTER(_SamplerConfigC), C.c_uint64,
                                  C.POINTER(C.POINTER(_SamplingResultC))]),
     "bk_result_free": (None, [C.POINTER(_SamplingResultC)]),
+    "bk_set_energy_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "bk_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "bk_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "bk_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTiming), C.c_int,
                                  C.POINTER(C.c_int)]),
@@ -150,6 +154,14 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with "
                               f"`make -C {HERE}` (there is no CPU fallback)")
+        # PyTorch-ROCm ships its own HIP runtime with the same SONAME
+        # (libamdhip64.so.7).  Loading torch first makes libbk bind to that
+        # copy, so one process holds one HIP runtime and torch tensors and
+        # libbk pointers share a device context.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
@@ -296,6 +308,14 @@ class Engine:
         _check(lib().bk_sample(self._h, _dp(r), r.shape[0],
                                out.ctypes.data_as(C.POINTER(C.c_int64))))
         return out
+
+    def set_energy_mode(self, mode: int):
+        """BK_ENERGY_CACHE16 (uint16 E(s) array) or BK_ENERGY_ONTHEFLY (popcounts)."""
+        _check(lib().bk_set_energy_mode(self._h, mode))
+
+    def set_tuning(self, key: str, value: int):
+        """Sweep knobs: 'grid', 'row_bits', 'pingpong'."""
+        _check(lib().bk_set_tuning(self._h, key.encode(), int(value)))
 
     # -- instrumentation
     def timing_enable(self, on: bool = True):

@@ -137,10 +137,14 @@ struct bk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf psi;
+    DevBuf psi_alt;  // second state buffer for out-of-place sweeps
     int n = 0;
     int precision = 0;
     bool has_state = false;
-    double pending_dtp = 0.0;  // closing Strang half-phase owed by the held state
+    // The stored state is exp(-i theta_pre E(s)/n) times the true state: the
+    // opening phase of the next Trotter step is applied by the last sweep of
+    // the previous one.  Exported states are corrected first (flush_phase).
+    double theta_pre = 0.0;
     DevBuf partials;           // 3 doubles per workgroup of the sweep
     DevBuf series;             // 3 doubles per measurement slot
     DevBuf count_partials;
@@ -149,6 +153,10 @@ struct bk_ctx {
     DevBuf blas_partials, blas_out;
     std::map<std::pair<int, std::pair<int, double>>, DevBuf> phase_tabs;  // (n, (prec, dtp))
     bk::SweepTuning tuning{};
+    int energy_mode = BK_ENERGY_CACHE16;  // how the LOW sweep gets E(s)
+    bool tuning_pinned = false;           // set by bk_set_tuning: keep across states
+    DevBuf ecache;                        // uint16 E(s) per amplitude
+    int ecache_n = 0;
     bool timing = false;
     EventTimer timer;
 };
@@ -218,22 +226,22 @@ int ensure_state(bk_ctx* c, int n, int precision)
     c->n = n;
     c->precision = precision;
     c->has_state = true;
-    c->pending_dtp = 0.0;
-    c->tuning = bk::default_tuning(n, precision);
+    c->theta_pre = 0.0;
+    if (!c->tuning_pinned) c->tuning = bk::default_tuning(n, precision);
     c->tuning.timer = c->timing ? &c->timer : nullptr;
     HIPCALL(c->partials.ensure(bk::partials_bytes(c->tuning.grid)));
     return BK_OK;
 }
 
-// apply any owed closing half-phase so the held state is the exact one
-int flush_pending(bk_ctx* c)
+// remove the pre-applied opening phase so the held state is the true one
+int flush_phase(bk_ctx* c)
 {
-    if (!c->has_state || c->pending_dtp == 0.0) return BK_OK;
+    if (!c->has_state || c->theta_pre == 0.0) return BK_OK;
     const void* tab = nullptr;
-    int rc = get_phase_tab(c, c->n, c->precision, c->pending_dtp, &tab);
+    int rc = get_phase_tab(c, c->n, c->precision, -c->theta_pre, &tab);
     if (rc != BK_OK) return rc;
-    HIPCALL(bk::launch_phase(c->psi.p, c->n, c->precision, tab, c->stream));
-    c->pending_dtp = 0.0;
+    HIPCALL(bk::launch_phase(c->psi.p, c->n, c->precision, tab, 0u, 0u, nullptr, nullptr, c->stream));
+    c->theta_pre = 0.0;
     return BK_OK;
 }
 
@@ -265,6 +273,7 @@ int bk_ctx_create(int device, bk_ctx** out)
     HIPCALL(hipSetDevice(device));
     bk_ctx* c = new bk_ctx();
     c->device = device;
+    c->tuning = bk::default_tuning(30, BK_C64);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -280,6 +289,7 @@ void bk_ctx_destroy(bk_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->psi.release();
+    c->psi_alt.release();
     c->partials.release();
     c->series.release();
     c->count_partials.release();
@@ -290,6 +300,7 @@ void bk_ctx_destroy(bk_ctx* c)
     c->samp_out.release();
     c->blas_partials.release();
     c->blas_out.release();
+    c->ecache.release();
     for (auto& kv : c->phase_tabs) kv.second.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -314,7 +325,7 @@ int bk_state_init(bk_ctx* c, int n, int precision, uint64_t start_bits)
     if (rc) return rc;
     if ((rc = ensure_state(c, n, precision))) return rc;
     const uint64_t s = start_bits & ((1ull << n) - 1ull);
-    HIPCALL(bk::launch_init_basis(c->psi.p, n, precision, s, c->stream));
+    HIPCALL(bk::launch_init_basis(c->psi.p, n, precision, s, 1.0, 0.0, c->stream));
     HIPCALL(hipStreamSynchronize(c->stream));
     return BK_OK;
 }
@@ -336,7 +347,7 @@ int bk_state_download(bk_ctx* c, void* host)
     if (rc) return rc;
     if (!host) return fail(BK_E_INVALID, "null host buffer");
     if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
-    if ((rc = flush_pending(c))) return rc;
+    if ((rc = flush_phase(c))) return rc;
     HIPCALL(hipMemcpyAsync(host, c->psi.p, amp_bytes(c->precision) << c->n, hipMemcpyDeviceToHost, c->stream));
     HIPCALL(hipStreamSynchronize(c->stream));
     return BK_OK;
@@ -347,7 +358,7 @@ int bk_state_device(bk_ctx* c, void** d, void** st, int* n, int* precision)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
-    if ((rc = flush_pending(c))) return rc;
+    if ((rc = flush_phase(c))) return rc;
     if (d) *d = c->psi.p;
     if (st) *st = c->stream;
     if (n) *n = c->n;
@@ -391,33 +402,85 @@ int bk_observables(bk_ctx* c, uint64_t s0, int64_t e0, double* out3)
     return BK_OK;
 }
 
-// Enqueue one Trotter step of the held state.  measure_slot: device pointer or null.
-static int enqueue_step(bk_ctx* c, int order, double dt, double g, uint64_t s0, uint32_t e0, double* slot)
+// Make the stored state carry the opening phase of a run's first step
+// (phi0 = dt for Lie, dt/2 for Strang).  fresh_start: the state is reset to
+// |s0> with that phase folded into its single amplitude, and the state-0
+// observables are exact (P_< = [E(s0) < e0], <d_H> = 0, norm = 1).  Otherwise
+// the missing phase is applied by one elementwise pass that also reduces the
+// state-0 observables into slot0 (or a read-only pass when no phase is missing).
+static int prepare_opening(bk_ctx* c, int order, double dt, bool fresh_start, uint64_t s0, uint32_t e0,
+                           double* slot0)
 {
-    double dtp;
-    if (order == BK_STRANG) {
-        dtp = c->pending_dtp + 0.5 * dt;  // merge the owed closing half with this opening half
-    } else {
-        if (c->pending_dtp != 0.0) {
-            int rc = flush_pending(c);
-            if (rc) return rc;
+    const double phi0 = (order == BK_STRANG) ? 0.5 * dt : dt;
+    if (fresh_start) {
+        // the state was just set to |s0> (bk_state_init): overwrite its amplitude
+        const double th = -(phi0 / static_cast<double>(c->n)) * static_cast<double>(bk::host_energy(c->n, s0));
+        const double re = std::cos(th), im = std::sin(th);
+        char* dst = static_cast<char*>(c->psi.p) + amp_bytes(c->precision) * s0;
+        HIPCALL(hipStreamSynchronize(c->stream));
+        if (c->precision == BK_C64) {
+            const float v[2] = {static_cast<float>(re), static_cast<float>(im)};
+            HIPCALL(hipMemcpy(dst, v, sizeof(v), hipMemcpyHostToDevice));
+        } else {
+            const double v[2] = {re, im};
+            HIPCALL(hipMemcpy(dst, v, sizeof(v), hipMemcpyHostToDevice));
         }
-        dtp = dt;
+        c->theta_pre = phi0;
+        if (slot0) {
+            const double h[3] = {bk::host_energy(c->n, s0) < static_cast<int64_t>(e0) ? 1.0 : 0.0, 0.0, 1.0};
+            HIPCALL(hipMemcpy(slot0, h, sizeof(h), hipMemcpyHostToDevice));
+        }
+        return BK_OK;
     }
+    const double need = phi0 - c->theta_pre;
+    if (need != 0.0) {
+        const void* tab = nullptr;
+        int rc = get_phase_tab(c, c->n, c->precision, need, &tab);
+        if (rc) return rc;
+        HIPCALL(bk::launch_phase(c->psi.p, c->n, c->precision, tab, static_cast<uint32_t>(s0), e0, slot0,
+                                 c->partials.as<double>(), c->stream));
+    } else if (slot0) {
+        HIPCALL(bk::launch_measure(c->psi.p, c->n, c->precision, static_cast<uint32_t>(s0), e0, slot0,
+                                   c->partials.as<double>(), c->tuning.grid, c->stream));
+    }
+    c->theta_pre = phi0;
+    return BK_OK;
+}
+
+// Enqueue one Trotter step of the held state: the rotations, then the phase
+// exp(-i dt E/n) -- for Lie the next step's opening phase, for Strang this
+// step's closing half plus the next step's opening half -- so theta_pre is
+// unchanged.  slot: device pointer for the stepped state's observables, or null.
+static int enqueue_step(bk_ctx* c, double dt, double g, uint64_t s0, uint32_t e0, double* slot)
+{
     const void* tab = nullptr;
-    int rc = get_phase_tab(c, c->n, c->precision, dtp, &tab);
+    int rc = get_phase_tab(c, c->n, c->precision, dt, &tab);
     if (rc) return rc;
     bk::StepSpec sp{};
     sp.n = c->n;
     sp.precision = c->precision;
     sp.gdt = g * dt;
-    sp.dtp = dtp;
+    sp.dtp = dt;
     sp.phase_tab = tab;
     sp.series_slot = slot;
     sp.s0 = static_cast<uint32_t>(s0);
     sp.e0 = e0;
-    HIPCALL(bk::launch_step(c->psi.p, sp, c->tuning, c->partials.as<double>(), c->stream));
-    c->pending_dtp = (order == BK_STRANG) ? 0.5 * dt : 0.0;
+    if (c->energy_mode == BK_ENERGY_CACHE16 && c->n >= 13) {
+        if (c->ecache_n != c->n) {
+            HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
+            HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->stream));
+            c->ecache_n = c->n;
+        }
+        sp.ecache = c->ecache.as<uint16_t>();
+    }
+    void* alt = nullptr;
+    if (c->tuning.pingpong && c->n >= 13) {
+        HIPCALL(c->psi_alt.ensure(amp_bytes(c->precision) << c->n));
+        alt = c->psi_alt.p;
+    }
+    int in_alt = 0;
+    HIPCALL(bk::launch_step(c->psi.p, alt, &in_alt, sp, c->tuning, c->partials.as<double>(), c->stream));
+    if (in_alt) std::swap(c->psi, c->psi_alt);
     return BK_OK;
 }
 
@@ -442,10 +505,15 @@ int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
         if (c->n != p->n || c->precision != p->precision)
             return fail(BK_E_INVALID, "held state is n=%d prec=%d", c->n, c->precision);
     }
-    // measurement slots
+    // measured state indices: 0 and multiples of measure_every, plus the last
     const int every = p->measure_every;
-    int64_t slots = 1;
-    if (every > 0) slots += (p->steps + every - 1) / every;
+    const int K = p->steps;
+    std::vector<int> meas;
+    if (every > 0) meas.push_back(0);
+    for (int st = 1; st <= K; ++st)
+        if ((every > 0 && st % every == 0) || st == K) meas.push_back(st);
+    if (meas.empty()) meas.push_back(0);  // K == 0 and every == 0: the state itself
+    const int64_t slots = static_cast<int64_t>(meas.size());
     if (out && out->capacity < slots && (out->p_less || out->dh || out->norm))
         return fail(BK_E_INVALID, "series capacity %lld < %lld", static_cast<long long>(out->capacity),
                     static_cast<long long>(slots));
@@ -453,14 +521,14 @@ int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
     double* ds = c->series.as<double>();
     const uint32_t e0u = static_cast<uint32_t>(std::max<int64_t>(0, std::min<int64_t>(e0, 0xFFFFFFFFll)));
     int64_t slot = 0;
-    for (int k = 0; k < p->steps; ++k) {
-        double* sl = nullptr;
-        if (every > 0 && k % every == 0) sl = ds + 3 * (slot++);
+    double* slot0 = (meas[0] == 0) ? ds : nullptr;
+    if ((rc = prepare_opening(c, p->order, p->dt, p->init_from_start != 0, s0, e0u, slot0))) return rc;
+    if (slot0) slot = 1;
+    for (int k = 0; k < K; ++k) {
+        double* sl = (slot < slots && meas[static_cast<size_t>(slot)] == k + 1) ? ds + 3 * (slot++) : nullptr;
         const double g = p->g_steps ? p->g_steps[k] : p->g;
-        if ((rc = enqueue_step(c, p->order, p->dt, g, s0, e0u, sl))) return rc;
+        if ((rc = enqueue_step(c, p->dt, g, s0, e0u, sl))) return rc;
     }
-    HIPCALL(bk::launch_measure(c->psi.p, c->n, c->precision, static_cast<uint32_t>(s0), e0u, ds + 3 * (slot++),
-                               c->partials.as<double>(), c->tuning.grid, c->stream));
     if (out) {
         std::vector<double> h(3 * static_cast<size_t>(slot));
         HIPCALL(hipMemcpyAsync(h.data(), ds, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -608,7 +676,7 @@ int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, dou
     if (!c->has_state) return fail(BK_E_NOSTATE, "no statevector held");
     if (c->precision != BK_C128) return fail(BK_E_INVALID, "Krylov propagation needs a complex128 state");
     if (m < 2 || !is_fin(g) || !is_fin(tau) || !(tau_restart > 0.0)) return fail(BK_E_INVALID, "bad argument");
-    if ((rc = flush_pending(c))) return rc;
+    if ((rc = flush_phase(c))) return rc;
     const int n = c->n;
     const int64_t dim = 1ll << n;
     DevBuf V, w, coef;
@@ -795,10 +863,14 @@ int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits,
     double* ds = c->series.as<double>();
     const uint32_t e0u = static_cast<uint32_t>(std::max<int64_t>(0, r->start_energy));
     int mi = 0, pi = 0;
+    // state 0 (|start> with the first step's opening phase folded in)
+    double* slot0 = (flags[0] & 1) ? ds : nullptr;
+    if ((rc = prepare_opening(c, order, cfg->dt, true, start, e0u, slot0))) return cleanup_fail(rc);
+    if (slot0) mi = 1;
     for (int step = 0; step <= total; ++step) {
         const unsigned char f = flags[static_cast<size_t>(step)];
         if (f & 1) tp.push_back(step * cfg->dt);
-        if (f & 2) {  // sampling reads |psi|^2 of the state before this step
+        if (f & 2) {  // sampling reads |psi|^2 of state 'step' (before this step's evolution)
             for (int s = 0; s < n_seeds; ++s) {
                 const size_t off = pi * per_point + static_cast<size_t>(s) * shots;
                 HIPCALL(bk::launch_sample(c->psi.p, n, precision, c->samp_rand.as<double>() + off, shots,
@@ -807,12 +879,10 @@ int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits,
             }
             ++pi;
         }
-        double* slot = (f & 1) ? ds + 3 * (mi++) : nullptr;
         if (step < total) {
-            if ((rc = enqueue_step(c, order, cfg->dt, cfg->g, start, e0u, slot))) return cleanup_fail(rc);
-        } else if (slot) {
-            HIPCALL(bk::launch_measure(c->psi.p, n, precision, static_cast<uint32_t>(start), e0u, slot,
-                                       c->partials.as<double>(), c->tuning.grid, c->stream));
+            // the step's last sweep reduces the observables of state step+1
+            double* slot = (flags[static_cast<size_t>(step) + 1] & 1) ? ds + 3 * (mi++) : nullptr;
+            if ((rc = enqueue_step(c, cfg->dt, cfg->g, start, e0u, slot))) return cleanup_fail(rc);
         }
     }
     std::vector<double> hs(3 * static_cast<size_t>(n_meas));
@@ -923,6 +993,49 @@ int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits,
                   << std::setprecision(4) << r->per_eval_rate << std::defaultfloat << "\n";
     }
     *out = guard.release();
+    return BK_OK;
+}
+
+int bk_set_tuning(bk_ctx* c, const char* key, int value)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!key) return fail(BK_E_INVALID, "null key");
+    const std::string k = key;
+    HIPCALL(hipStreamSynchronize(c->stream));
+    if (k == "grid") {
+        if (value < 1 || value > (1 << 20)) return fail(BK_E_INVALID, "grid %d", value);
+        c->tuning.grid = value;
+        HIPCALL(c->partials.ensure(bk::partials_bytes(value)));
+    } else if (k == "row_bits") {
+        if (value < 3 || value > 8) return fail(BK_E_INVALID, "row_bits %d", value);
+        c->tuning.row_bits = value;
+    } else if (k == "prefetch_low") {
+        c->tuning.prefetch_low = value != 0;
+    } else if (k == "prefetch_high") {
+        c->tuning.prefetch_high = value != 0;
+    } else if (k == "pingpong") {
+        c->tuning.pingpong = value != 0;
+        if (!value) c->psi_alt.release();
+    } else {
+        return fail(BK_E_INVALID, "unknown tuning key '%s'", key);
+    }
+    c->tuning_pinned = true;
+    return BK_OK;
+}
+
+int bk_set_energy_mode(bk_ctx* c, int mode)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (mode != BK_ENERGY_ONTHEFLY && mode != BK_ENERGY_CACHE16)
+        return fail(BK_E_INVALID, "energy mode %d", mode);
+    c->energy_mode = mode;
+    if (mode == BK_ENERGY_ONTHEFLY) {
+        HIPCALL(hipStreamSynchronize(c->stream));
+        c->ecache.release();
+        c->ecache_n = 0;
+    }
     return BK_OK;
 }
 

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <utility>
 
 namespace bk {
 
@@ -117,12 +118,14 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // ---------------------------------------------------------------------------
 // Tiled sweep kernel
 // ---------------------------------------------------------------------------
-enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4 };
+enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16 };
 
 struct SweepArgs {
     uint64_t ntiles;
     const void* tab;   // phase table (V per energy) for F_PHASE
     double* partials;  // gridDim.x * 3 doubles for F_MEASURE
+    const uint16_t* ecache;  // F_ECACHE: E(s) per amplitude (exact, < 2^16)
+    void* dst;               // output state (null: in place)
     double bc, bs;     // butterfly coefficients (F_TAN: bs = tan, bc unused)
     double scale;      // overall factor applied with the phase (c^n in F_TAN form)
     int n;
@@ -141,7 +144,7 @@ struct LowPlan {
 };
 
 template <typename V, class PH, int FLAGS, int SW>
-__global__ __launch_bounds__(kThreads) void k_sweep(V* __restrict__ psi, const SweepArgs a)
+__global__ __launch_bounds__(kThreads) void k_sweep(V* psi, const SweepArgs a)
 {
     using R = typename RealOf<V>::T;
     constexpr Plan P = PH::plan;
@@ -169,46 +172,44 @@ __global__ __launch_bounds__(kThreads) void k_sweep(V* __restrict__ psi, const S
     const uint64_t tgL = gl(tu[NP - 1]);
     const R bc = static_cast<R>(a.bc), bs = static_cast<R>(a.bs);
     double acc[3] = {0.0, 0.0, 0.0};
+    constexpr bool PF = (FLAGS & F_PREFETCH) != 0;
+    constexpr int NEC = (FLAGS & F_ECACHE) ? NR : 1;
 
-    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-        uint64_t base;
+    auto tile_base = [&](uint64_t tile) -> uint64_t {
         if constexpr (LOW) {
-            base = tile << TB;
+            return tile << TB;
         } else {
             const uint64_t lo = tile & ((1ull << a.lo_bits) - 1ull);
             const uint64_t hi = tile >> a.lo_bits;
-            base = (lo << RR) | (hi << (a.h + TB - RR));
+            return (lo << RR) | (hi << (a.h + TB - RR));
         }
-        V x[NR];
-        V* tp = psi + base;
+    };
+    // phase-0 layout loads of one tile (amplitudes and, with the cache, E(s))
+    auto load = [&](uint64_t base, V(&xv)[NR], uint32_t(&ev)[NEC]) {
+        const V* tp = psi + base;
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             const uint32_t pj = reg_pattern(P.ph[0], RB, j);
-            x[j] = tp[tg0 + gl(pj)];
+            xv[j] = tp[tg0 + gl(pj)];
+            // energies are consumed after the last phase: load them in that layout
+            if constexpr ((FLAGS & F_ECACHE) != 0)
+                ev[j] = a.ecache[base + tgL + reg_pattern(P.ph[NP - 1], RB, j)];
         }
-        if constexpr ((FLAGS & (F_PHASE | F_MEASURE)) != 0) {
-            static_assert(LOW, "phase/measure only in the LOW sweep");
-            const V* tab = static_cast<const V*>(a.tab);
-            const R sc = static_cast<R>(a.scale);
-#pragma unroll
-            for (int j = 0; j < NR; ++j) {
-                const uint32_t s = static_cast<uint32_t>(base + tg0 + reg_pattern(P.ph[0], RB, j));
-                const uint32_t E = labs_energy_u32(s, a.n);
-                if constexpr ((FLAGS & F_MEASURE) != 0) {
-                    const double xr = static_cast<double>(x[j].x), xi = static_cast<double>(x[j].y);
-                    const double pr = fma(xr, xr, xi * xi);
-                    acc[2] += pr;
-                    acc[0] += (E < a.e0) ? pr : 0.0;
-                    acc[1] = fma(pr, static_cast<double>(__popc(s ^ a.s0)), acc[1]);
-                }
-                if constexpr ((FLAGS & F_PHASE) != 0) {
-                    const V ph = tab[E];
-                    const R pr = ph.x * sc, pi = ph.y * sc;
-                    const R xr = x[j].x, xi = x[j].y;
-                    x[j].x = xr * pr - xi * pi;
-                    x[j].y = xr * pi + xi * pr;
-                }
-            }
+    };
+
+    V x[NR];
+    uint32_t ec[NEC];
+    uint64_t tile = blockIdx.x;
+    if (tile < a.ntiles) load(tile_base(tile), x, ec);
+    for (; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile_base(tile);
+        const uint64_t nxt = tile + gridDim.x;
+        // software pipelining: the next tile's loads are issued before this
+        // tile's stores, so waiting for them never waits on the stores
+        V xn[PF ? NR : 1];
+        uint32_t ecn[PF ? NEC : 1];
+        if constexpr (PF) {
+            if (nxt < a.ntiles) load(tile_base(nxt), xn, ecn);
         }
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
@@ -232,10 +233,51 @@ __global__ __launch_bounds__(kThreads) void k_sweep(V* __restrict__ psi, const S
                 }
             }
         }
+        // after the last rotation of the step: scale (c^n in tan form) and the
+        // next step's phase exp(-i dt E/n), then P_< / <d_H> / norm of the
+        // stepped state (|psi|^2 does not see the phase)
+        if constexpr ((FLAGS & (F_PHASE | F_MEASURE)) != 0) {
+            static_assert(LOW, "phase/measure only in the LOW sweep");
+            const V* tab = static_cast<const V*>(a.tab);
+            const R sc = static_cast<R>(a.scale);
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                const uint32_t s = static_cast<uint32_t>(base + tgL + reg_pattern(P.ph[NP - 1], RB, j));
+                uint32_t E;
+                if constexpr ((FLAGS & F_ECACHE) != 0) E = ec[j];
+                else E = labs_energy_u32(s, a.n);
+                if constexpr ((FLAGS & F_PHASE) != 0) {
+                    const V ph = tab[E];
+                    const R pr = ph.x * sc, pi = ph.y * sc;
+                    const R xr = x[j].x, xi = x[j].y;
+                    x[j].x = xr * pr - xi * pi;
+                    x[j].y = xr * pi + xi * pr;
+                }
+                if constexpr ((FLAGS & F_MEASURE) != 0) {
+                    const double xr = static_cast<double>(x[j].x), xi = static_cast<double>(x[j].y);
+                    const double pr = fma(xr, xr, xi * xi);
+                    acc[2] += pr;
+                    acc[0] += (E < a.e0) ? pr : 0.0;
+                    acc[1] = fma(pr, static_cast<double>(__popc(s ^ a.s0)), acc[1]);
+                }
+            }
+        }
+        V* op = (a.dst ? static_cast<V*>(a.dst) : psi) + base;
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             const uint32_t pj = reg_pattern(P.ph[NP - 1], RB, j);
-            tp[tgL + gl(pj)] = x[j];
+            op[tgL + gl(pj)] = x[j];
+        }
+        if (nxt < a.ntiles) {
+            if constexpr (PF) {
+#pragma unroll
+                for (int j = 0; j < NR; ++j) x[j] = xn[j];
+                if constexpr ((FLAGS & F_ECACHE) != 0)
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) ec[j] = ecn[j];
+            } else {
+                load(tile_base(nxt), x, ec);
+            }
         }
     }
     if constexpr ((FLAGS & F_MEASURE) != 0) {
@@ -286,13 +328,6 @@ __global__ __launch_bounds__(kThreads) void k_elementwise(V* __restrict__ psi, u
         V x = psi[i];
         const uint32_t s = static_cast<uint32_t>(i);
         const uint32_t E = labs_energy_u32(s, n);
-        if constexpr ((FLAGS & F_MEASURE) != 0) {
-            const double xr = static_cast<double>(x.x), xi = static_cast<double>(x.y);
-            const double pr = fma(xr, xr, xi * xi);
-            acc[2] += pr;
-            acc[0] += (E < e0) ? pr : 0.0;
-            acc[1] = fma(pr, static_cast<double>(__popc(s ^ s0)), acc[1]);
-        }
         if constexpr ((FLAGS & F_PHASE) != 0) {
             const V ph = tab[E];
             const R pr = ph.x * sc, pi = ph.y * sc;
@@ -300,6 +335,13 @@ __global__ __launch_bounds__(kThreads) void k_elementwise(V* __restrict__ psi, u
             x.x = xr * pr - xi * pi;
             x.y = xr * pi + xi * pr;
             psi[i] = x;
+        }
+        if constexpr ((FLAGS & F_MEASURE) != 0) {
+            const double xr = static_cast<double>(x.x), xi = static_cast<double>(x.y);
+            const double pr = fma(xr, xr, xi * xi);
+            acc[2] += pr;
+            acc[0] += (E < e0) ? pr : 0.0;
+            acc[1] = fma(pr, static_cast<double>(__popc(s ^ s0)), acc[1]);
         }
     }
     if constexpr ((FLAGS & F_MEASURE) != 0) block_reduce_store<3>(acc, scratch, partials + 3 * blockIdx.x);
@@ -320,12 +362,20 @@ __global__ __launch_bounds__(kThreads) void k_finish3(const double* __restrict__
 }
 
 template <typename V>
-__global__ void k_set_basis(V* psi, uint64_t s0)
+__global__ void k_set_basis(V* psi, uint64_t s0, double re, double im)
 {
-    V one;
-    one.x = 1;
-    one.y = 0;
-    psi[s0] = one;
+    V v;
+    v.x = static_cast<typename RealOf<V>::T>(re);
+    v.y = static_cast<typename RealOf<V>::T>(im);
+    psi[s0] = v;
+}
+
+// E(s) for every s, stored as uint16 (E <= sum_{j<n} j^2 = 10416 at n = 32)
+__global__ __launch_bounds__(kThreads) void k_energy_fill(uint16_t* __restrict__ out, int n, uint64_t dim)
+{
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        out[i] = static_cast<uint16_t>(labs_energy_u32(static_cast<uint32_t>(i), n));
 }
 
 // number of s in [0, 2^n) with E(s) < e0
@@ -422,11 +472,25 @@ hipError_t launch_low(V* psi, const SweepArgs& a, int grid, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename V, int R, bool TAN>
+// runtime flag set -> template instantiation of the LOW sweep
+template <typename V, int FL = 0>
+hipError_t launch_low_dyn(int fl, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (FL >= 32) {
+        return hipErrorInvalidValue;
+    } else if constexpr ((FL & F_PHASE) == 0) {
+        return launch_low_dyn<V, FL + 1>(fl, psi, a, grid, st);  // LOW always carries the phase
+    } else {
+        if (fl == FL) return launch_low<V, FL>(psi, a, grid, st);
+        return launch_low_dyn<V, FL + 1>(fl, psi, a, grid, st);
+    }
+}
+
+template <typename V, int R, bool TAN, bool PF>
 hipError_t launch_high_r(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    auto k = k_sweep<V, HighPlan<C::TB, C::RB, R>, TAN ? F_TAN : 0, C::SW>;
+    auto k = k_sweep<V, HighPlan<C::TB, C::RB, R>, (TAN ? F_TAN : 0) | (PF ? F_PREFETCH : 0), C::SW>;
     const size_t lds = sizeof(V) << C::TB;
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
@@ -434,15 +498,24 @@ hipError_t launch_high_r(V* psi, const SweepArgs& a, int grid, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename V, bool TAN, int R = 3>
+template <typename V, bool TAN, bool PF, int R = 3>
 hipError_t launch_high(V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
 {
     if constexpr (R > Cfg<V>::TB - 1) {
         return hipErrorInvalidValue;
     } else {
-        if (rbits == R) return launch_high_r<V, R, TAN>(psi, a, grid, st);
-        return launch_high<V, TAN, R + 1>(psi, rbits, a, grid, st);
+        if (rbits == R) return launch_high_r<V, R, TAN, PF>(psi, a, grid, st);
+        return launch_high<V, TAN, PF, R + 1>(psi, rbits, a, grid, st);
     }
+}
+
+template <typename V>
+hipError_t launch_high_dyn(bool tan, bool pf, V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if (tan) return pf ? launch_high<V, true, true>(psi, rbits, a, grid, st)
+                       : launch_high<V, true, false>(psi, rbits, a, grid, st);
+    return pf ? launch_high<V, false, true>(psi, rbits, a, grid, st)
+              : launch_high<V, false, false>(psi, rbits, a, grid, st);
 }
 
 int clamp_grid(uint64_t ntiles, int grid)
@@ -452,9 +525,12 @@ int clamp_grid(uint64_t ntiles, int grid)
 }
 
 template <typename V>
-hipError_t step_impl(V* psi, const StepSpec& sp, const SweepTuning& tun, double* partials,
-                     hipStream_t st)
+hipError_t step_impl(V* psi, V* alt, int* in_alt, const StepSpec& sp, const SweepTuning& tun,
+                     double* partials, hipStream_t st)
 {
+    // Step order: HIGH sweeps (qubits [TB, n)), then the LOW sweep (qubits
+    // [0, TB)) which ends with scale * exp(-i dtp E/n) and the reductions.
+    *in_alt = 0;
     using C = Cfg<V>;
     const int n = sp.n;
     const MixCoef mc = mix_coef(n, sp.gdt);
@@ -462,7 +538,18 @@ hipError_t step_impl(V* psi, const StepSpec& sp, const SweepTuning& tun, double*
     const bool measure = sp.series_slot != nullptr;
     hipError_t e = hipSuccess;
     if (n < C::TB) {
-        // small states: elementwise phase (+measure) then one kernel per qubit
+        // small states: one kernel per qubit, then elementwise scale*phase (+measure)
+        const uint64_t np = dim / 2;
+        const int qg = static_cast<int>(std::min<uint64_t>((np + kThreads - 1) / kThreads, 4096));
+        for (int q = 0; q < n; ++q) {
+            if (mc.tan_form)
+                hipLaunchKernelGGL((k_qubit<V, true>), dim3(qg), dim3(kThreads), 0, st, psi, q, np,
+                                   mc.bc, mc.bs);
+            else
+                hipLaunchKernelGGL((k_qubit<V, false>), dim3(qg), dim3(kThreads), 0, st, psi, q, np,
+                                   mc.bc, mc.bs);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         const uint64_t chunk = 4096;
         const int grid = static_cast<int>((dim + chunk - 1) / chunk);
         if (measure)
@@ -476,17 +563,6 @@ hipError_t step_impl(V* psi, const StepSpec& sp, const SweepTuning& tun, double*
             hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, grid, sp.series_slot);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
-        const uint64_t np = dim / 2;
-        const int qg = static_cast<int>(std::min<uint64_t>((np + kThreads - 1) / kThreads, 4096));
-        for (int q = 0; q < n; ++q) {
-            if (mc.tan_form)
-                hipLaunchKernelGGL((k_qubit<V, true>), dim3(qg), dim3(kThreads), 0, st, psi, q, np,
-                                   mc.bc, mc.bs);
-            else
-                hipLaunchKernelGGL((k_qubit<V, false>), dim3(qg), dim3(kThreads), 0, st, psi, q, np,
-                                   mc.bc, mc.bs);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-        }
         return hipSuccess;
     }
     SweepArgs a{};
@@ -498,43 +574,31 @@ hipError_t step_impl(V* psi, const StepSpec& sp, const SweepTuning& tun, double*
     a.n = n;
     a.s0 = sp.s0;
     a.e0 = sp.e0;
-    // LOW sweep: phase (+ measurement) and qubits [0, TB)
+    a.ecache = sp.ecache;
     a.ntiles = dim >> C::TB;
-    const int g1 = clamp_grid(a.ntiles, tun.grid);
     const double sweep_bytes = 2.0 * static_cast<double>(sizeof(V)) * static_cast<double>(dim);
-    if (tun.timer) tun.timer->begin(st);
-    if (measure) {
-        e = mc.tan_form ? launch_low<V, F_PHASE | F_MEASURE | F_TAN>(psi, a, g1, st)
-                        : launch_low<V, F_PHASE | F_MEASURE>(psi, a, g1, st);
-    } else {
-        e = mc.tan_form ? launch_low<V, F_PHASE | F_TAN>(psi, a, g1, st)
-                        : launch_low<V, F_PHASE>(psi, a, g1, st);
-    }
-    if (e != hipSuccess) return e;
-    if (tun.timer) tun.timer->end(st, "sweep_low", sweep_bytes);
-    if (measure) {
-        hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, g1, sp.series_slot);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    // HIGH sweeps over the remaining qubits [TB, n)
+    // out-of-place sweeps alternate between psi and alt when alt is given
+    V* cur = psi;
+    V* oth = alt;
+    // HIGH sweeps over qubits [TB, n)
     const int rmin = std::max(tun.row_bits, 3);
     const int amax = C::TB - rmin;
     const int remaining = n - C::TB;
-    if (remaining <= 0) return hipSuccess;
-    const int nh = (remaining + amax - 1) / amax;
+    const int nh = remaining > 0 ? (remaining + amax - 1) / amax : 0;
     int h = C::TB;
     for (int i = 0; i < nh; ++i) {
         const int left = n - h;
         const int A = (left + (nh - i) - 1) / (nh - i);  // even split, larger first
         const int R = C::TB - A;
         SweepArgs b = a;
+        b.dst = alt ? oth : nullptr;
         b.h = h;
         b.lo_bits = h - R;
-        b.ntiles = dim >> C::TB;
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
-        e = mc.tan_form ? launch_high<V, true>(psi, R, b, g, st) : launch_high<V, false>(psi, R, b, g, st);
+        e = launch_high_dyn<V>(mc.tan_form, tun.prefetch_high, cur, R, b, g, st);
         if (e != hipSuccess) return e;
+        if (alt) std::swap(cur, oth);
         if (tun.timer) {
             char name[48];
             std::snprintf(name, sizeof(name), "sweep_high_q%d_%d", h, h + A - 1);
@@ -542,6 +606,21 @@ hipError_t step_impl(V* psi, const StepSpec& sp, const SweepTuning& tun, double*
         }
         h += A;
     }
+    // LOW sweep: qubits [0, TB), then scale*phase and the reductions
+    const int g1 = clamp_grid(a.ntiles, tun.grid);
+    a.dst = alt ? oth : nullptr;
+    const int fl = F_PHASE | (measure ? F_MEASURE : 0) | (mc.tan_form ? F_TAN : 0) |
+                   (sp.ecache ? F_ECACHE : 0) | (tun.prefetch_low ? F_PREFETCH : 0);
+    if (tun.timer) tun.timer->begin(st);
+    e = launch_low_dyn<V>(fl, cur, a, g1, st);
+    if (e != hipSuccess) return e;
+    if (alt) std::swap(cur, oth);
+    if (tun.timer) tun.timer->end(st, "sweep_low", sweep_bytes);
+    if (measure) {
+        hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, g1, sp.series_slot);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    *in_alt = (cur != psi);
     return hipSuccess;
 }
 
@@ -553,18 +632,25 @@ SweepTuning default_tuning(int n, int precision)
     SweepTuning t{};
     t.tile_bits = precision == kC64 ? kLowTB64 : kLowTB128;
     t.row_bits = precision == kC64 ? 4 : 3;
-    t.grid = 2048;
+    t.grid = 4096;
+    t.pingpong = false;
+    t.prefetch_low = false;
+    t.prefetch_high = false;
     return t;
 }
 
 size_t partials_bytes(int grid) { return sizeof(double) * 3 * static_cast<size_t>(std::max(grid, 4096)); }
 
-hipError_t launch_step(void* psi, const StepSpec& spec, const SweepTuning& tun, double* partials,
-                       hipStream_t stream)
+hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepSpec& spec,
+                       const SweepTuning& tun, double* partials, hipStream_t stream)
 {
     if (spec.n < 1 || spec.n > 32) return hipErrorInvalidValue;
-    if (spec.precision == kC64) return step_impl(static_cast<float2*>(psi), spec, tun, partials, stream);
-    if (spec.precision == kC128) return step_impl(static_cast<double2*>(psi), spec, tun, partials, stream);
+    if (spec.precision == kC64)
+        return step_impl(static_cast<float2*>(psi), static_cast<float2*>(psi_alt), result_in_alt, spec, tun,
+                         partials, stream);
+    if (spec.precision == kC128)
+        return step_impl(static_cast<double2*>(psi), static_cast<double2*>(psi_alt), result_in_alt, spec, tun,
+                         partials, stream);
     return hipErrorInvalidValue;
 }
 
@@ -598,32 +684,54 @@ hipError_t launch_measure(const void* psi, int n, int precision, uint32_t s0, ui
 }
 
 template <typename V>
-static hipError_t phase_impl(V* psi, int n, const void* tab, hipStream_t st)
+static hipError_t phase_impl(V* psi, int n, const void* tab, uint32_t s0, uint32_t e0, double* slot,
+                             double* partials, hipStream_t st)
 {
     const uint64_t dim = 1ull << n;
     uint64_t chunk = 4096;
-    while ((dim + chunk - 1) / chunk > 65536) chunk <<= 1;
+    while ((dim + chunk - 1) / chunk > 4096) chunk <<= 1;
     const int grid = static_cast<int>((dim + chunk - 1) / chunk);
-    hipLaunchKernelGGL((k_elementwise<V, F_PHASE>), dim3(grid), dim3(kThreads), 0, st, psi, dim, chunk, tab,
-                       1.0, n, 0u, 0u, nullptr);
+    if (slot) {
+        hipLaunchKernelGGL((k_elementwise<V, F_PHASE | F_MEASURE>), dim3(grid), dim3(kThreads), 0, st, psi, dim,
+                           chunk, tab, 1.0, n, s0, e0, partials);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, grid, slot);
+    } else {
+        hipLaunchKernelGGL((k_elementwise<V, F_PHASE>), dim3(grid), dim3(kThreads), 0, st, psi, dim, chunk, tab,
+                           1.0, n, 0u, 0u, nullptr);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, hipStream_t stream)
+hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, uint32_t s0, uint32_t e0,
+                        double* series_slot, double* partials, hipStream_t stream)
 {
-    if (precision == kC64) return phase_impl(static_cast<float2*>(psi), n, phase_tab, stream);
-    if (precision == kC128) return phase_impl(static_cast<double2*>(psi), n, phase_tab, stream);
+    if (precision == kC64)
+        return phase_impl(static_cast<float2*>(psi), n, phase_tab, s0, e0, series_slot, partials, stream);
+    if (precision == kC128)
+        return phase_impl(static_cast<double2*>(psi), n, phase_tab, s0, e0, series_slot, partials, stream);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, hipStream_t stream)
+hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, double re, double im,
+                             hipStream_t stream)
 {
     const size_t esz = precision == kC64 ? sizeof(float2) : sizeof(double2);
     hipError_t e = hipMemsetAsync(psi, 0, esz << n, stream);
     if (e != hipSuccess) return e;
     if (precision == kC64) hipLaunchKernelGGL(k_set_basis<float2>, dim3(1), dim3(1), 0, stream,
-                                              static_cast<float2*>(psi), s0);
-    else hipLaunchKernelGGL(k_set_basis<double2>, dim3(1), dim3(1), 0, stream, static_cast<double2*>(psi), s0);
+                                              static_cast<float2*>(psi), s0, re, im);
+    else hipLaunchKernelGGL(k_set_basis<double2>, dim3(1), dim3(1), 0, stream, static_cast<double2*>(psi), s0,
+                            re, im);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy_fill(uint16_t* out, int n, hipStream_t stream)
+{
+    const uint64_t dim = 1ull << n;
+    const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 65536));
+    hipLaunchKernelGGL(k_energy_fill, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
     return hipGetLastError();
 }
 

@@ -20,14 +20,17 @@ struct StepSpec {
     int precision;
     // mixer: per-qubit [[c, i s],[i s, c]], c = cos(g dt), s = sin(g dt)
     double gdt;
-    // diagonal phase applied in the first sweep before the rotations:
-    // exp(-i dtp E(s)/n); dtp == 0 -> no phase
+    // diagonal phase applied after the step's rotations (the next step's
+    // opening phase): exp(-i dtp E(s)/n)
     double dtp;
     const void* phase_tab;  // device table indexed by E (float2 or double2)
-    // measurement of the incoming state (before the phase) into slot
+    // observables of the stepped state reduced into slot
     double* series_slot;    // device pointer to 3 doubles {P_<, <d_H>, norm} or null
     uint32_t s0;
     uint32_t e0;
+    // optional uint16 energy cache (E(s) per amplitude); null -> energies are
+    // computed on the fly with popcounts in the sweep
+    const uint16_t* ecache = nullptr;
 };
 
 // Optional per-launch timing hook (HIP events recorded on the launch stream).
@@ -42,6 +45,9 @@ struct SweepTuning {
     int tile_bits;   // TB of the LOW sweep (13 for c64, 12 for c128 by default)
     int row_bits;    // R of HIGH sweeps
     int grid;        // workgroups per sweep launch (persistent loop over tiles)
+    bool pingpong;   // out-of-place sweeps between two state buffers
+    bool prefetch_low;   // software-pipelined tile loads in the LOW sweep
+    bool prefetch_high;  // ... and in the HIGH sweeps
     KernelTimer* timer = nullptr;
 };
 
@@ -50,19 +56,27 @@ SweepTuning default_tuning(int n, int precision);
 // scratch for reductions (partials): bytes needed for a given grid
 size_t partials_bytes(int grid);
 
-// Launch one Trotter step (phase + all rotations) on 'stream'.
-hipError_t launch_step(void* psi, const StepSpec& spec, const SweepTuning& tun, double* partials,
-                       hipStream_t stream);
+// Launch one Trotter step (phase + all rotations) on 'stream'.  With psi_alt
+// non-null the sweeps run out of place, alternating psi -> psi_alt -> psi ...;
+// *result_in_alt tells which buffer holds the stepped state.
+hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepSpec& spec,
+                       const SweepTuning& tun, double* partials, hipStream_t stream);
 
 // Read-only measurement of the state into series_slot (3 doubles).
 hipError_t launch_measure(const void* psi, int n, int precision, uint32_t s0, uint32_t e0,
                           double* series_slot, double* partials, int grid, hipStream_t stream);
 
-// Phase only (exp(-i dtp E/n)), e.g. the closing half step of Strang.
-hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, hipStream_t stream);
+// Elementwise phase psi_s *= tab[E(s)]; with series_slot non-null the
+// observables of the resulting state are reduced into it in the same pass.
+hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, uint32_t s0, uint32_t e0,
+                        double* series_slot, double* partials, hipStream_t stream);
 
-// psi = |s0>
-hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, hipStream_t stream);
+// psi = (re + i im) |s0>
+hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, double re, double im,
+                             hipStream_t stream);
+
+// out[s] = E(s) for all 2^n states (uint16, exact for n <= 32)
+hipError_t launch_energy_fill(uint16_t* out, int n, hipStream_t stream);
 
 // count of s with E(s) < e0 over all 2^n states (exact, integer)
 hipError_t launch_count_below(int n, uint32_t e0, unsigned long long* d_count,

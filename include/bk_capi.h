@@ -202,6 +202,24 @@ int bk_sampler_run(bk_ctx* ctx, const bk_sampler_config* cfg, uint64_t start_bit
                    bk_sampling_result** out);
 void bk_result_free(bk_sampling_result* r);
 
+/* ---- how the first sweep of a step obtains E(s) (the reference caches E as FP64,
+ * 8 B per amplitude, sampler.cuh:54-70, 320-338) ----
+ * BK_ENERGY_CACHE16 (default): a uint16 E(s) array (2 B/amplitude, exact) filled
+ *   once per n and read by the sweep;
+ * BK_ENERGY_ONTHEFLY: recomputed every step from popcounts (no extra memory). */
+#define BK_ENERGY_ONTHEFLY 0
+#define BK_ENERGY_CACHE16 1
+int bk_set_energy_mode(bk_ctx* ctx, int mode);
+
+/* ---- sweep tuning knobs (no reference counterpart; used by the autotuner) ----
+ * "grid"     : workgroups per sweep launch (persistent loop over tiles)
+ * "row_bits" : contiguous low bits carried by the high-qubit sweeps (3..8)
+ * "pingpong" : 1 = out-of-place sweeps between two state buffers (2x memory)
+ * "prefetch_low" / "prefetch_high" : 1 = issue the next tile's loads before the
+ *              current tile's stores (software pipelining) in that sweep kind
+ * Values persist for the context until changed. */
+int bk_set_tuning(bk_ctx* ctx, const char* key, int value);
+
 /* ---- instrumentation (no reference counterpart; the reference only wall-times
  * the whole sample() call, connector.cuh:627, 832-833) ----
  * When enabled, every sweep launch of bk_evolve / bk_sampler_run is bracketed

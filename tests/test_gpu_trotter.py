@@ -136,3 +136,39 @@ def test_c64_against_c128_n24(engine, bk):
     assert np.max(np.abs(a["p_less"] - b["p_less"])) <= 1e-5
     assert np.max(np.abs(a["dh"] - b["dh"])) <= 1e-4
     assert 1.0 - abs(np.vdot(psi128, psi64)) <= 1e-5
+
+
+@pytest.mark.parametrize("n,prec", [(14, C128), (20, C64), (22, C128)])
+def test_energy_modes_bit_identical(engine, orc, bk, n, prec):
+    """uint16 energy cache and on-the-fly popcount energies feed the same integer
+    E(s) to the same kernels: the series must agree bit for bit."""
+    _, s0, e0 = orc.make_trap(n, 31)
+    engine.set_energy_mode(bk.BK_ENERGY_ONTHEFLY)
+    a = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
+    psi_a = engine.download()
+    engine.set_energy_mode(bk.BK_ENERGY_CACHE16)
+    b = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
+    psi_b = engine.download()
+    assert np.array_equal(a["p_less"], b["p_less"]) and np.array_equal(a["dh"], b["dh"])
+    assert np.array_equal(psi_a, psi_b)
+
+
+@pytest.mark.parametrize("n,prec", [(17, C128), (22, C64), (25, C64)])
+def test_pingpong_and_grid_bit_identical(engine, orc, bk, n, prec):
+    """Out-of-place sweeps and other persistent grid sizes change only where
+    data lives, not the arithmetic: results must match the in-place run exactly."""
+    _, s0, e0 = orc.make_trap(n, 41)
+    base = engine.evolve(n, s0, 0.35, 0.02, 9, order=STRANG, precision=prec)
+    psi0 = engine.download()
+    try:
+        for key, val in (("pingpong", 1), ("grid", 333), ("grid", 100000)):
+            engine.set_tuning(key, val)
+            got = engine.evolve(n, s0, 0.35, 0.02, 9, order=STRANG, precision=prec)
+            psi = engine.download()
+            assert np.array_equal(psi, psi0), (key, val)
+            # the FP64 reduction order follows the grid; values agree to rounding
+            assert np.max(np.abs(got["p_less"] - base["p_less"])) <= 1e-15
+            assert np.max(np.abs(got["dh"] - base["dh"])) <= 1e-13
+    finally:
+        engine.set_tuning("pingpong", 0)
+        engine.set_tuning("grid", 2048)
