@@ -134,6 +134,7 @@ _SIGS = {
     "bk_result_free": (None, [C.POINTER(_SamplingResultC)]),
     "bk_set_energy_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "bk_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "bk_sweep_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     "bk_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "bk_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTiming), C.c_int,
                                  C.POINTER(C.c_int)]),
@@ -205,6 +206,13 @@ def make_trap(n: int, seed: int):
     a, b, e = C.c_uint64(), C.c_uint64(), C.c_int64()
     _check(lib().bk_make_trap(n, seed, C.byref(a), C.byref(b), C.byref(e)))
     return a.value, b.value, e.value
+
+
+def sweep_plan(n: int, precision: int = BK_C64, row_bits: int = 5, page_bits: int = 0):
+    """HIGH sweeps of one Trotter step: list of (h1, a1, h2, a2, R)."""
+    buf = (C.c_int * 40)()
+    k = _check(lib().bk_sweep_plan(n, precision, row_bits, page_bits, buf, 8))
+    return [tuple(buf[5 * i:5 * i + 5]) for i in range(k)]
 
 
 def make_trap_min_dmin(n: int, seed: int, min_dmin: int, max_tries: int = 100000):

@@ -25,6 +25,7 @@
 #include "../../include/bk_capi.h"
 #include "bk_host.hpp"
 #include "bk_kernels.hpp"
+#include "bk_sweeps.hpp"
 
 namespace {
 
@@ -155,8 +156,9 @@ struct bk_ctx {
     bk::SweepTuning tuning{};
     int energy_mode = BK_ENERGY_CACHE16;  // how the LOW sweep gets E(s)
     bool tuning_pinned = false;           // set by bk_set_tuning: keep across states
-    DevBuf ecache;                        // uint16 E(s) per amplitude
+    DevBuf ecache;                        // uint16 E(s) per amplitude (LOW tile layout)
     int ecache_n = 0;
+    int ecache_prec = 0;
     bool timing = false;
     EventTimer timer;
 };
@@ -182,25 +184,30 @@ int get_phase_tab(bk_ctx* c, int n, int precision, double dtp, const void** out)
         *out = it->second.p;
         return BK_OK;
     }
+    // C_k = (n-k) mod 2, so every E(s) = 4m + c with c = #{k : n-k odd} mod 4:
+    // entry m holds the phase of E = 4m + c and kernels index it by E >> 2.
     const int64_t emax = static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6;  // sum_{j<n} j^2
-    const size_t cnt = static_cast<size_t>(emax) + 1;
+    int64_t codd = 0;
+    for (int k = 1; k < n; ++k) codd += (n - k) & 1;
+    const int64_t c4 = codd % 4;
+    const size_t cnt = static_cast<size_t>(emax >> 2) + 1;
     const double w = -(dtp / static_cast<double>(n));
     DevBuf buf;
     if (precision == BK_C64) {
         std::vector<float> h(2 * cnt);
-        for (size_t e = 0; e < cnt; ++e) {
-            const double th = w * static_cast<double>(e);
-            h[2 * e] = static_cast<float>(std::cos(th));
-            h[2 * e + 1] = static_cast<float>(std::sin(th));
+        for (size_t m = 0; m < cnt; ++m) {
+            const double th = w * static_cast<double>(4 * static_cast<int64_t>(m) + c4);
+            h[2 * m] = static_cast<float>(std::cos(th));
+            h[2 * m + 1] = static_cast<float>(std::sin(th));
         }
         HIPCALL(buf.ensure(h.size() * sizeof(float)));
         HIPCALL(hipMemcpy(buf.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     } else {
         std::vector<double> h(2 * cnt);
-        for (size_t e = 0; e < cnt; ++e) {
-            const double th = w * static_cast<double>(e);
-            h[2 * e] = std::cos(th);
-            h[2 * e + 1] = std::sin(th);
+        for (size_t m = 0; m < cnt; ++m) {
+            const double th = w * static_cast<double>(4 * static_cast<int64_t>(m) + c4);
+            h[2 * m] = std::cos(th);
+            h[2 * m + 1] = std::sin(th);
         }
         HIPCALL(buf.ensure(h.size() * sizeof(double)));
         HIPCALL(hipMemcpy(buf.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -466,10 +473,11 @@ static int enqueue_step(bk_ctx* c, double dt, double g, uint64_t s0, uint32_t e0
     sp.s0 = static_cast<uint32_t>(s0);
     sp.e0 = e0;
     if (c->energy_mode == BK_ENERGY_CACHE16 && c->n >= 13) {
-        if (c->ecache_n != c->n) {
+        if (c->ecache_n != c->n || c->ecache_prec != c->precision) {
             HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
-            HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->stream));
+            HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream));
             c->ecache_n = c->n;
+            c->ecache_prec = c->precision;
         }
         sp.ecache = c->ecache.as<uint16_t>();
     }
@@ -1010,6 +1018,9 @@ int bk_set_tuning(bk_ctx* c, const char* key, int value)
     } else if (k == "row_bits") {
         if (value < 3 || value > 8) return fail(BK_E_INVALID, "row_bits %d", value);
         c->tuning.row_bits = value;
+    } else if (k == "page_bits") {
+        if (value < 0 || value > 32) return fail(BK_E_INVALID, "page_bits %d", value);
+        c->tuning.page_bits = value;
     } else if (k == "prefetch_low") {
         c->tuning.prefetch_low = value != 0;
     } else if (k == "prefetch_high") {
@@ -1037,6 +1048,23 @@ int bk_set_energy_mode(bk_ctx* c, int mode)
         c->ecache_n = 0;
     }
     return BK_OK;
+}
+
+int bk_sweep_plan(int n, int precision, int row_bits, int page_bits, int* out, int cap)
+{
+    if (n < 1 || n > BK_MAX_N || (precision != BK_C64 && precision != BK_C128) || (cap > 0 && !out))
+        return fail(BK_E_INVALID, "bk_sweep_plan: bad argument");
+    const int tb = precision == BK_C64 ? 13 : 12, htb = precision == BK_C64 ? 14 : 13;
+    if (n < tb) return 0;  // small states use per-qubit kernels
+    const std::vector<bk::HighSweep> pl = bk::plan_high_sweeps(n, tb, htb, std::max(row_bits, 3), page_bits);
+    for (size_t i = 0; i < pl.size() && static_cast<int>(i) < cap; ++i) {
+        out[5 * i + 0] = pl[i].h1;
+        out[5 * i + 1] = pl[i].a1;
+        out[5 * i + 2] = pl[i].h2;
+        out[5 * i + 3] = pl[i].a2;
+        out[5 * i + 4] = pl[i].R;
+    }
+    return static_cast<int>(pl.size());
 }
 
 int bk_timing_enable(bk_ctx* c, int enable)

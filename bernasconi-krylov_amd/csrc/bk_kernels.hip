@@ -18,11 +18,14 @@
 // |cos(g dt)| < 0.5 the full (c, s) butterfly is used instead.
 #include "bk_kernels.hpp"
 #include "bk_plan.hpp"
+#include "bk_sweeps.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <numeric>
 #include <utility>
+#include <vector>
 
 namespace bk {
 
@@ -129,8 +132,14 @@ struct SweepArgs {
     double bc, bs;     // butterfly coefficients (F_TAN: bs = tan, bc unused)
     double scale;      // overall factor applied with the phase (c^n in F_TAN form)
     int n;
-    int h;             // HIGH: global bit of local bit R
-    int lo_bits;       // HIGH: tile-index bits placed at global [R, h)
+    // HIGH: the active local bits [R, TB) map to two global segments,
+    // [h, h+a1) then [h2, h2+(TB-R-a1)); the tile index fills the remaining
+    // bits [R, h), [h+a1, h2), [h2+a2, n) in that order.
+    int h;
+    int a1;
+    int h2;
+    int lo_bits;       // = h - R
+    int mid_bits;      // = h2 - (h + a1)
     uint32_t s0, e0;
 };
 
@@ -144,11 +153,12 @@ struct LowPlan {
 };
 
 template <typename V, class PH, int FLAGS, int SW>
-__global__ __launch_bounds__(kThreads) void k_sweep(V* psi, const SweepArgs a)
+__global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const SweepArgs a)
 {
     using R = typename RealOf<V>::T;
     constexpr Plan P = PH::plan;
-    constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R;
+    constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R, TT = P.TT();
+    static_assert(!P.low || TT == kThreadBits, "the LOW sweep runs 256 threads");
     constexpr bool LOW = P.low;
     constexpr uint32_t ROWMASK = (1u << RR) - 1u;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -160,28 +170,36 @@ __global__ __launch_bounds__(kThreads) void k_sweep(V* psi, const SweepArgs a)
     for (int p = 0; p < NP; ++p) {
         uint32_t u = 0;
 #pragma unroll
-        for (int b = 0; b < kThreadBits; ++b) u |= ((t >> b) & 1u) << P.ph[p].thr[b];
+        for (int b = 0; b < TT; ++b) u |= ((t >> b) & 1u) << P.ph[p].thr[b];
         tu[p] = u;
     }
     // global offset of a local index (HIGH: rows stay, active bits move to h)
     auto gl = [&](uint32_t u) -> uint64_t {
         if constexpr (LOW) return u;
-        else return static_cast<uint64_t>(u & ROWMASK) + (static_cast<uint64_t>(u >> RR) << a.h);
+        else {
+            const uint32_t ua = u >> RR;
+            return static_cast<uint64_t>(u & ROWMASK) + (static_cast<uint64_t>(ua & ((1u << a.a1) - 1u)) << a.h) +
+                   (static_cast<uint64_t>(ua >> a.a1) << a.h2);
+        }
     };
     const uint64_t tg0 = gl(tu[0]);
     const uint64_t tgL = gl(tu[NP - 1]);
     const R bc = static_cast<R>(a.bc), bs = static_cast<R>(a.bs);
     double acc[3] = {0.0, 0.0, 0.0};
     constexpr bool PF = (FLAGS & F_PREFETCH) != 0;
-    constexpr int NEC = (FLAGS & F_ECACHE) ? NR : 1;
+    // energy cache words: the cache is stored tile by tile in the last phase's
+    // register layout (thread t's NR values contiguous), two uint16 per word
+    constexpr int NEC = (FLAGS & F_ECACHE) ? NR / 2 : 1;
+    static_assert(!(FLAGS & F_ECACHE) || (NR % 8 == 0), "cache rows are whole 16-byte words");
 
     auto tile_base = [&](uint64_t tile) -> uint64_t {
         if constexpr (LOW) {
             return tile << TB;
         } else {
             const uint64_t lo = tile & ((1ull << a.lo_bits) - 1ull);
-            const uint64_t hi = tile >> a.lo_bits;
-            return (lo << RR) | (hi << (a.h + TB - RR));
+            const uint64_t mid = (tile >> a.lo_bits) & ((1ull << a.mid_bits) - 1ull);
+            const uint64_t hi = tile >> (a.lo_bits + a.mid_bits);
+            return (lo << RR) | (mid << (a.h + a.a1)) | (hi << (a.h2 + (TB - RR - a.a1)));
         }
     };
     // phase-0 layout loads of one tile (amplitudes and, with the cache, E(s))
@@ -191,9 +209,17 @@ __global__ __launch_bounds__(kThreads) void k_sweep(V* psi, const SweepArgs a)
         for (int j = 0; j < NR; ++j) {
             const uint32_t pj = reg_pattern(P.ph[0], RB, j);
             xv[j] = tp[tg0 + gl(pj)];
-            // energies are consumed after the last phase: load them in that layout
-            if constexpr ((FLAGS & F_ECACHE) != 0)
-                ev[j] = a.ecache[base + tgL + reg_pattern(P.ph[NP - 1], RB, j)];
+        }
+        if constexpr ((FLAGS & F_ECACHE) != 0) {
+            const uint4* ep = reinterpret_cast<const uint4*>(a.ecache + base + static_cast<uint64_t>(t) * NR);
+#pragma unroll
+            for (int q = 0; q < NR / 8; ++q) {
+                const uint4 w = ep[q];
+                ev[4 * q + 0] = w.x;
+                ev[4 * q + 1] = w.y;
+                ev[4 * q + 2] = w.z;
+                ev[4 * q + 3] = w.w;
+            }
         }
     };
 
@@ -240,26 +266,32 @@ __global__ __launch_bounds__(kThreads) void k_sweep(V* psi, const SweepArgs a)
             static_assert(LOW, "phase/measure only in the LOW sweep");
             const V* tab = static_cast<const V*>(a.tab);
             const R sc = static_cast<R>(a.scale);
+            R tacc[3] = {R(0), R(0), R(0)};
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
                 const uint32_t s = static_cast<uint32_t>(base + tgL + reg_pattern(P.ph[NP - 1], RB, j));
                 uint32_t E;
-                if constexpr ((FLAGS & F_ECACHE) != 0) E = ec[j];
+                if constexpr ((FLAGS & F_ECACHE) != 0) E = (ec[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
                 else E = labs_energy_u32(s, a.n);
                 if constexpr ((FLAGS & F_PHASE) != 0) {
-                    const V ph = tab[E];
+                    const V ph = tab[E >> 2];  // E = 4m + (n/2 mod 4): one entry per m
                     const R pr = ph.x * sc, pi = ph.y * sc;
                     const R xr = x[j].x, xi = x[j].y;
                     x[j].x = xr * pr - xi * pi;
                     x[j].y = xr * pi + xi * pr;
                 }
                 if constexpr ((FLAGS & F_MEASURE) != 0) {
-                    const double xr = static_cast<double>(x[j].x), xi = static_cast<double>(x[j].y);
-                    const double pr = fma(xr, xr, xi * xi);
-                    acc[2] += pr;
-                    acc[0] += (E < a.e0) ? pr : 0.0;
-                    acc[1] = fma(pr, static_cast<double>(__popc(s ^ a.s0)), acc[1]);
+                    // per-tile partials in the state's precision (a thread's
+                    // NR terms), folded into FP64 once per tile
+                    const R pr = fma(x[j].x, x[j].x, x[j].y * x[j].y);
+                    tacc[2] += pr;
+                    tacc[0] += (E < a.e0) ? pr : R(0);
+                    tacc[1] = fma(pr, static_cast<R>(__popc(s ^ a.s0)), tacc[1]);
                 }
+            }
+            if constexpr ((FLAGS & F_MEASURE) != 0) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) acc[i] += static_cast<double>(tacc[i]);
             }
         }
         V* op = (a.dst ? static_cast<V*>(a.dst) : psi) + base;
@@ -274,7 +306,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(V* psi, const SweepArgs a)
                 for (int j = 0; j < NR; ++j) x[j] = xn[j];
                 if constexpr ((FLAGS & F_ECACHE) != 0)
 #pragma unroll
-                    for (int j = 0; j < NR; ++j) ec[j] = ecn[j];
+                    for (int j = 0; j < NEC; ++j) ec[j] = ecn[j];
             } else {
                 load(tile_base(nxt), x, ec);
             }
@@ -329,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void k_elementwise(V* __restrict__ psi, u
         const uint32_t s = static_cast<uint32_t>(i);
         const uint32_t E = labs_energy_u32(s, n);
         if constexpr ((FLAGS & F_PHASE) != 0) {
-            const V ph = tab[E];
+            const V ph = tab[E >> 2];
             const R pr = ph.x * sc, pi = ph.y * sc;
             const R xr = x.x, xi = x.y;
             x.x = xr * pr - xi * pi;
@@ -370,12 +402,25 @@ __global__ void k_set_basis(V* psi, uint64_t s0, double re, double im)
     psi[s0] = v;
 }
 
-// E(s) for every s, stored as uint16 (E <= sum_{j<n} j^2 = 10416 at n = 32)
+// E(s) for every s as uint16 (E <= sum_{j<n} j^2 = 10416 at n = 32), stored in
+// the LOW sweep's order: tile by tile, thread t's NR values of its last-phase
+// registers contiguous, so the sweep reads them with 16-byte loads.
+template <class PH>
 __global__ __launch_bounds__(kThreads) void k_energy_fill(uint16_t* __restrict__ out, int n, uint64_t dim)
 {
+    constexpr Plan P = PH::plan;
+    constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP;
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        out[i] = static_cast<uint16_t>(labs_energy_u32(static_cast<uint32_t>(i), n));
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t tile = i >> TB;
+        const uint32_t t = static_cast<uint32_t>(i >> RB) & ((1u << (TB - RB)) - 1u);
+        const uint32_t j = static_cast<uint32_t>(i) & (NR - 1u);
+        uint32_t u = 0;
+        for (int b = 0; b < TB - RB; ++b) u |= ((t >> b) & 1u) << P.ph[NP - 1].thr[b];
+        for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
+        const uint64_t s = (tile << TB) | u;
+        out[i] = static_cast<uint16_t>(labs_energy_u32(static_cast<uint32_t>(s), n));
+    }
 }
 
 // number of s in [0, 2^n) with E(s) < e0
@@ -416,15 +461,20 @@ constexpr int kLowTB64 = 13, kLowRB64 = 5;
 constexpr int kLowTB128 = 12, kLowRB128 = 4;
 constexpr int kSW64 = 5, kSW128 = 4;
 
+// LOW tiles: 2^TB amplitudes, 256 threads.  HIGH tiles: 2^HTB amplitudes,
+// 2^(HTB-HRB) threads and 2^HTB * sizeof(V) bytes of LDS (128 KB), so their
+// contiguous rows can be 256-512 B while each sweep still rotates 8-9 qubits.
 template <typename V>
 struct Cfg;
 template <>
 struct Cfg<float2> {
     static constexpr int TB = kLowTB64, RB = kLowRB64, SW = kSW64;
+    static constexpr int HTB = 14, HRB = 5;
 };
 template <>
 struct Cfg<double2> {
     static constexpr int TB = kLowTB128, RB = kLowRB128, SW = kSW128;
+    static constexpr int HTB = 13, HRB = 4;
 };
 
 struct MixCoef {
@@ -490,18 +540,19 @@ template <typename V, int R, bool TAN, bool PF>
 hipError_t launch_high_r(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    auto k = k_sweep<V, HighPlan<C::TB, C::RB, R>, (TAN ? F_TAN : 0) | (PF ? F_PREFETCH : 0), C::SW>;
-    const size_t lds = sizeof(V) << C::TB;
+    using PH = HighPlan<C::HTB, C::HRB, R>;
+    auto k = k_sweep<V, PH, (TAN ? F_TAN : 0) | (PF ? F_PREFETCH : 0), C::SW>;
+    const size_t lds = sizeof(V) << C::HTB;
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, psi, a);
     return hipGetLastError();
 }
 
 template <typename V, bool TAN, bool PF, int R = 3>
 hipError_t launch_high(V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
 {
-    if constexpr (R > Cfg<V>::TB - 1) {
+    if constexpr (R > Cfg<V>::HTB - 1) {
         return hipErrorInvalidValue;
     } else {
         if (rbits == R) return launch_high_r<V, R, TAN, PF>(psi, a, grid, st);
@@ -581,30 +632,31 @@ hipError_t step_impl(V* psi, V* alt, int* in_alt, const StepSpec& sp, const Swee
     V* cur = psi;
     V* oth = alt;
     // HIGH sweeps over qubits [TB, n)
-    const int rmin = std::max(tun.row_bits, 3);
-    const int amax = C::TB - rmin;
-    const int remaining = n - C::TB;
-    const int nh = remaining > 0 ? (remaining + amax - 1) / amax : 0;
-    int h = C::TB;
-    for (int i = 0; i < nh; ++i) {
-        const int left = n - h;
-        const int A = (left + (nh - i) - 1) / (nh - i);  // even split, larger first
-        const int R = C::TB - A;
+    const std::vector<HighSweep> plan = plan_high_sweeps(n, C::TB, C::HTB, std::max(tun.row_bits, 3),
+                                                         tun.page_bits);
+    for (const HighSweep& hs : plan) {
         SweepArgs b = a;
         b.dst = alt ? oth : nullptr;
-        b.h = h;
-        b.lo_bits = h - R;
+        b.h = hs.h1;
+        b.a1 = hs.a1;
+        b.h2 = hs.h2;
+        b.lo_bits = hs.h1 - hs.R;
+        b.mid_bits = hs.h2 - (hs.h1 + hs.a1);
+        b.ntiles = dim >> C::HTB;
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
-        e = launch_high_dyn<V>(mc.tan_form, tun.prefetch_high, cur, R, b, g, st);
+        e = launch_high_dyn<V>(mc.tan_form, tun.prefetch_high, cur, hs.R, b, g, st);
         if (e != hipSuccess) return e;
         if (alt) std::swap(cur, oth);
         if (tun.timer) {
             char name[48];
-            std::snprintf(name, sizeof(name), "sweep_high_q%d_%d", h, h + A - 1);
+            if (hs.a2 > 0)
+                std::snprintf(name, sizeof(name), "sweep_high_q%d_%d+q%d_%d", hs.h1, hs.h1 + hs.a1 - 1, hs.h2,
+                              hs.h2 + hs.a2 - 1);
+            else
+                std::snprintf(name, sizeof(name), "sweep_high_q%d_%d", hs.h1, hs.h1 + hs.a1 - 1);
             tun.timer->end(st, name, sweep_bytes);
         }
-        h += A;
     }
     // LOW sweep: qubits [0, TB), then scale*phase and the reductions
     const int g1 = clamp_grid(a.ntiles, tun.grid);
@@ -631,11 +683,12 @@ SweepTuning default_tuning(int n, int precision)
     (void)n;
     SweepTuning t{};
     t.tile_bits = precision == kC64 ? kLowTB64 : kLowTB128;
-    t.row_bits = precision == kC64 ? 4 : 3;
+    t.row_bits = precision == kC64 ? 5 : 4;
     t.grid = 4096;
     t.pingpong = false;
     t.prefetch_low = false;
     t.prefetch_high = false;
+    t.page_bits = precision == kC64 ? 20 : 19;
     return t;
 }
 
@@ -727,11 +780,16 @@ hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, doubl
     return hipGetLastError();
 }
 
-hipError_t launch_energy_fill(uint16_t* out, int n, hipStream_t stream)
+hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream)
 {
     const uint64_t dim = 1ull << n;
     const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 65536));
-    hipLaunchKernelGGL(k_energy_fill, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
+    auto k64 = k_energy_fill<LowPlan<Cfg<float2>::TB, Cfg<float2>::RB>>;
+    auto k128 = k_energy_fill<LowPlan<Cfg<double2>::TB, Cfg<double2>::RB>>;
+    if (precision == kC64)
+        hipLaunchKernelGGL(k64, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
+    else
+        hipLaunchKernelGGL(k128, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
     return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ struct StepSpec {
     // diagonal phase applied after the step's rotations (the next step's
     // opening phase): exp(-i dtp E(s)/n)
     double dtp;
-    const void* phase_tab;  // device table indexed by E (float2 or double2)
+    const void* phase_tab;  // device table indexed by E >> 2 (float2 or double2)
     // observables of the stepped state reduced into slot
     double* series_slot;    // device pointer to 3 doubles {P_<, <d_H>, norm} or null
     uint32_t s0;
@@ -48,6 +48,7 @@ struct SweepTuning {
     bool pingpong;   // out-of-place sweeps between two state buffers
     bool prefetch_low;   // software-pipelined tile loads in the LOW sweep
     bool prefetch_high;  // ... and in the HIGH sweeps
+    int page_bits;       // > 0: deal qubits >= page_bits evenly over the HIGH sweeps
     KernelTimer* timer = nullptr;
 };
 
@@ -75,8 +76,9 @@ hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, 
 hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, double re, double im,
                              hipStream_t stream);
 
-// out[s] = E(s) for all 2^n states (uint16, exact for n <= 32)
-hipError_t launch_energy_fill(uint16_t* out, int n, hipStream_t stream);
+// E(s) for all 2^n states (uint16, exact for n <= 32) in the LOW sweep's tile
+// layout for 'precision' (the layout differs between c64 and c128)
+hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream);
 
 // count of s with E(s) < e0 over all 2^n states (exact, integer)
 hipError_t launch_count_below(int n, uint32_t e0, unsigned long long* d_count,

@@ -1,10 +1,10 @@
 // bk_plan.hpp -- compile-time tile plans for the LDS-tiled sigma_x sweeps.
 //
 // A sweep kernel moves one tile of 2^TB amplitudes per workgroup iteration.
-// The tile's local index u has TB bits; the 256 threads and the 2^RB
+// The tile's local index u has TB bits; the 2^(TB-RB) threads and the 2^RB
 // registers each thread holds cover those bits.  A *phase* fixes which local
 // bits sit in the register index (reg[0..RB)) and which in the thread index
-// (thr[0..8)); butterflies on register bits need no communication.  Between
+// (thr[0..TB-RB)); butterflies on register bits need no communication.  Between
 // phases the tile is exchanged through LDS so that another set of local bits
 // lands in registers.  'act' marks the register bits whose qubits are rotated
 // in that phase.
@@ -21,13 +21,14 @@
 
 namespace bk {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;     // workgroup size of the LOW sweep and the small kernels
 constexpr int kThreadBits = 8;
 constexpr int kMaxPhases = 4;
+constexpr int kMaxBits = 10;      // register or thread bits per phase
 
 struct Phase {
-    int reg[8];
-    int thr[8];
+    int reg[kMaxBits];
+    int thr[kMaxBits];
     unsigned act;  // bitmask over register index bits
 };
 
@@ -35,6 +36,7 @@ struct Plan {
     int TB, RB, R, NP;
     bool low;
     Phase ph[kMaxPhases];
+    constexpr int TT() const { return TB - RB; }  // thread bits: 2^TT threads per workgroup
 };
 
 // local-bit set helpers (TB <= 16)
@@ -46,11 +48,11 @@ constexpr Phase make_phase(const int* regbits, int RB, unsigned act_mask_local, 
         p.reg[i] = regbits[i];
         used |= 1u << regbits[i];
     }
-    for (int i = RB; i < 8; ++i) p.reg[i] = -1;
+    for (int i = RB; i < kMaxBits; ++i) p.reg[i] = -1;
     int k = 0;
     for (int b = 0; b < TB; ++b)
         if (!(used & (1u << b))) p.thr[k++] = b;
-    for (; k < 8; ++k) p.thr[k] = -1;
+    for (; k < kMaxBits; ++k) p.thr[k] = -1;
     p.act = 0;
     for (int i = 0; i < RB; ++i)
         if (act_mask_local & (1u << regbits[i])) p.act |= 1u << i;
@@ -68,7 +70,7 @@ constexpr Plan make_low_plan(int TB, int RB)
     int np = 0;
     unsigned done = 0;
     {
-        int regs[8] = {};
+        int regs[kMaxBits] = {};
         for (int i = 0; i < RB; ++i) regs[i] = TB - RB + i;
         unsigned act = 0;
         for (int i = 0; i < RB; ++i) act |= 1u << regs[i];
@@ -77,7 +79,7 @@ constexpr Plan make_low_plan(int TB, int RB)
     }
     int next = 0;  // remaining bits [0, TB-RB), chunked from the bottom
     while (next < TB - RB) {
-        int regs[8] = {};
+        int regs[kMaxBits] = {};
         int k = 0;
         unsigned act = 0;
         while (k < RB && next < TB - RB) {
@@ -115,7 +117,7 @@ constexpr Plan make_high_plan(int TB, int RB, int R)
     unsigned done = 0;
     int next = R;
     while (next < TB) {
-        int regs[8] = {};
+        int regs[kMaxBits] = {};
         int k = 0;
         unsigned act = 0;
         while (k < RB && next < TB) {

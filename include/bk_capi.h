@@ -215,10 +215,20 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  * "grid"     : workgroups per sweep launch (persistent loop over tiles)
  * "row_bits" : contiguous low bits carried by the high-qubit sweeps (3..8)
  * "pingpong" : 1 = out-of-place sweeps between two state buffers (2x memory)
+ * "page_bits": > 0 deals the qubits >= page_bits (whose partner amplitudes lie
+ *              in other 2^page_bits-amplitude pages) evenly over the high-qubit
+ *              sweeps instead of giving them all to the last one (0 = contiguous)
  * "prefetch_low" / "prefetch_high" : 1 = issue the next tile's loads before the
  *              current tile's stores (software pipelining) in that sweep kind
  * Values persist for the context until changed. */
 int bk_set_tuning(bk_ctx* ctx, const char* key, int value);
+
+/* The HIGH sweeps one Trotter step runs for (n, precision, row_bits, page_bits):
+ * writes {h1, a1, h2, a2, R} per sweep into out (5*cap ints) and returns the
+ * sweep count.  Qubits [0, 13) (c64) / [0, 12) (c128) belong to the final LOW
+ * sweep; a HIGH sweep rotates qubits [h1, h1+a1) and [h2, h2+a2) on tiles of
+ * 2^R-amplitude rows.  Host-only (no device needed). */
+int bk_sweep_plan(int n, int precision, int row_bits, int page_bits, int* out, int cap);
 
 /* ---- instrumentation (no reference counterpart; the reference only wall-times
  * the whole sample() call, connector.cuh:627, 832-833) ----

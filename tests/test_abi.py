@@ -87,3 +87,27 @@ def test_factory_above_max_n_raises(bk):
     s = bk.create_sampler(33)
     with pytest.raises(RuntimeError):
         s.sample(0, bk.SamplerConfig(n=33))
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_sweep_plan_covers_every_qubit_once(bk, prec):
+    """Host planner of the HIGH sweeps: every qubit in [TB, n) is rotated by
+    exactly one sweep, rows stay below the active bits, every tile index stays
+    inside the 2^n state."""
+    tb, htb = (13, 14) if prec == 1 else (12, 13)
+    for n in range(1, 33):
+        for rb in (3, 4, 5, 6, 7):
+            for pb in (0, 15, 17, 18, 19, 20, 24):
+                plan = bk.sweep_plan(n, prec, rb, pb)
+                if n <= tb:
+                    assert plan == []
+                    continue
+                seen = []
+                for h1, a1, h2, a2, R in plan:
+                    A = a1 + a2
+                    assert R + A == htb and R >= max(rb, 3) and a1 >= 1
+                    assert R <= h1 and h1 + a1 <= h2 and h2 + a2 <= n
+                    seen += list(range(h1, h1 + a1)) + list(range(h2, h2 + a2))
+                    # tile index fields: [R,h1), [h1+a1,h2), [h2+a2,n) sum to n-htb bits
+                    assert (h1 - R) + (h2 - h1 - a1) + (n - h2 - a2) == n - htb
+                assert sorted(seen) == list(range(tb, n)), (n, rb, pb, plan)
