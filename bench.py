@@ -42,14 +42,20 @@ def _cpu_baseline(n: int, seed: int, threads: int):
     }
 
 
-def _traffic():
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+def _traffic(kernel: str, n: int, precision: str, energy: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel calibration), for
+    the workload that summary was measured on; None otherwise."""
     path = os.path.join(ROOT, "profiles", "traffic_n30_c64.json")
-    if not os.path.exists(path):
+    if not (os.path.exists(path) and n == 30 and precision == "c64" and energy == "cache16"):
         return None
     with open(path) as f:
-        d = json.load(f)
-    return d.get("traffic_bytes_per_launch")
+        d = json.load(f)["kernels"]
+    if kernel == "sweep_low":
+        return d["sweep_low"]["traffic_bytes_per_launch"]
+    if kernel.startswith("sweep_high"):
+        return max(v["traffic_bytes_per_launch"] for k, v in d.items() if k.startswith("sweep_high"))
+    return None
 
 
 def main() -> int:
@@ -66,6 +72,8 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=30)
     ap.add_argument("--energy", choices=["cache16", "onthefly"], default="cache16",
                     help="how the first sweep obtains E(s)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain 8 GiB device copy (HBM ceiling / PMC calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -75,6 +83,11 @@ def main() -> int:
         return 2
 
     bk = importlib.import_module("bernasconi-krylov_amd")
+    if args.copy_probe:
+        eng = bk.Engine(0)
+        print(json.dumps({"copy_probe_bytes": 1 << 33, "gbs_read_plus_write": eng.copy_probe(1 << 33, 5)}))
+        eng.close()
+        return 0
     prec = bk.BK_C64 if args.precision == "c64" else bk.BK_C128
     n = args.n
     start, s0, e0 = bk.make_trap(n, args.seed)
@@ -112,6 +125,8 @@ def main() -> int:
                            "gbs": round(r["bytes_per_launch"] / (r["total_ms"] / r["launches"])
                                         * 1e-6, 1)} for r in rows}
 
+    copy_gbs = eng.copy_probe(1 << 33, 3)  # practical read+write ceiling on this device
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -148,7 +163,9 @@ def main() -> int:
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": _traffic(),
+            "traffic": _traffic(dom["name"], n, args.precision, args.energy),
+            "copy_ceiling_gbs": copy_gbs,
+            "frac_of_copy_ceiling": achieved / copy_gbs,
         },
         "cpu_baseline": cpu,
     }

@@ -136,6 +136,7 @@ _SIGS = {
     "bk_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "bk_sweep_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     "bk_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "bk_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "bk_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTiming), C.c_int,
                                  C.POINTER(C.c_int)]),
 }
@@ -336,6 +337,12 @@ class Engine:
         return [{"name": buf[i].name.decode(), "launches": buf[i].launches,
                  "total_ms": buf[i].total_ms, "bytes_per_launch": buf[i].bytes_per_launch}
                 for i in range(cnt.value)]
+
+    def copy_probe(self, nbytes: int = 1 << 33, reps: int = 5) -> float:
+        """GB/s (read+write) of a plain device copy of nbytes."""
+        g = C.c_double()
+        _check(lib().bk_copy_probe(self._h, nbytes, reps, C.byref(g)))
+        return g.value
 
     # -- subspace
     def hv(self, n: int, g: float, x, y, normalize_by_n: bool = True, stream=None):

@@ -1067,6 +1067,33 @@ int bk_sweep_plan(int n, int precision, int row_bits, int page_bits, int* out, i
     return static_cast<int>(pl.size());
 }
 
+int bk_copy_probe(bk_ctx* c, uint64_t bytes, int reps, double* gbs)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (bytes < 16 || bytes % 16 || reps < 1 || !gbs) return fail(BK_E_INVALID, "bk_copy_probe: bad argument");
+    DevBuf a, b;
+    HIPCALL(a.ensure(bytes));
+    HIPCALL(b.ensure(bytes));
+    HIPCALL(hipMemsetAsync(a.p, 0, bytes, c->stream));
+    HIPCALL(bk::launch_copy_probe(a.p, b.p, bytes, c->stream));  // warm
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCALL(hipEventCreate(&e0));
+    HIPCALL(hipEventCreate(&e1));
+    HIPCALL(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCALL(bk::launch_copy_probe(a.p, b.p, bytes, c->stream));
+    HIPCALL(hipEventRecord(e1, c->stream));
+    HIPCALL(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCALL(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9;
+    a.release();
+    b.release();
+    return BK_OK;
+}
+
 int bk_timing_enable(bk_ctx* c, int enable)
 {
     int rc = check_ctx(c);

@@ -111,4 +111,7 @@ hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V,
                           const double* coef, double* y, hipStream_t stream);
 size_t blas_partials_doubles();
 
+// plain float4 copy of 'bytes' (multiple of 16): HBM ceiling probe
+hipError_t launch_copy_probe(const void* src, void* dst, uint64_t bytes, hipStream_t stream);
+
 }  // namespace bk

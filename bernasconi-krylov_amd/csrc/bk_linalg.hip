@@ -472,4 +472,23 @@ hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V, c
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Plain copy (HBM ceiling probe and PMC calibration: known bytes per launch)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b,
+                                                    uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        b[i] = a[i];
+}
+
+hipError_t launch_copy_probe(const void* src, void* dst, uint64_t bytes, hipStream_t st)
+{
+    const uint64_t n = bytes / sizeof(float4);
+    hipLaunchKernelGGL(k_copy_probe, dim3(16384), dim3(256), 0, st, static_cast<const float4*>(src),
+                       static_cast<float4*>(dst), n);
+    return hipGetLastError();
+}
+
 }  // namespace bk

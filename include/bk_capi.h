@@ -244,6 +244,10 @@ typedef struct {
 } bk_kernel_timing;
 
 int bk_timing_enable(bk_ctx* ctx, int enable);
+/* Plain float4 copy of 'bytes' (read + write), reps times: returns GB/s of
+ * read+write traffic.  The practical HBM ceiling for the sweeps' access mix,
+ * and the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE. */
+int bk_copy_probe(bk_ctx* ctx, uint64_t bytes, int reps, double* gbs);
 int bk_timing_read(bk_ctx* ctx, bk_kernel_timing* out, int cap, int* count);
 
 #ifdef __cplusplus

@@ -111,3 +111,47 @@ def test_sweep_plan_covers_every_qubit_once(bk, prec):
                     # tile index fields: [R,h1), [h1+a1,h2), [h2+a2,n) sum to n-htb bits
                     assert (h1 - R) + (h2 - h1 - a1) + (n - h2 - a2) == n - htb
                 assert sorted(seen) == list(range(tb, n)), (n, rb, pb, plan)
+
+
+_CXX_PROBE = r"""
+#include "bk/sampler.hpp"
+#include "bk/diagonalizer.hpp"
+#include <cstdio>
+int main() {
+    SamplerConfig c;
+    std::printf("cfg %d %g %g %g %d %d %llu\n", c.n, c.g, c.dt, c.t_max, c.num_sample_points,
+                c.shots_per_point, (unsigned long long)c.seed);
+    DiagConfig d;
+    std::printf("diag %d %d %g\n", d.num_eigenpairs, d.max_krylov_dim, d.tolerance);
+    auto big = create_sampler(33);
+    try { big->sample(SpinState(0), c); return 1; }
+    catch (const std::runtime_error& e) { std::printf("big ok\n"); }
+    if (bk_device_count() == 0) {
+        try { auto s = create_sampler(16); return 2; }
+        catch (const std::runtime_error& e) { std::printf("nodev ok\n"); }
+    }
+    return 0;
+}
+"""
+
+
+def test_cxx_dropin_headers_compile_and_link(bk, tmp_path):
+    """include/bk/*.hpp compile as C++17 with -Wall -Werror, link against
+    libbk.so and keep the reference's SamplerConfig / DiagConfig defaults."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "probe.cpp"
+    src.write_text(_CXX_PROBE)
+    exe = tmp_path / "probe"
+    libdir = os.path.dirname(bk.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-Wno-unused-variable",
+                    "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir, "-lbk",
+                    "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    assert lines[0] == "cfg 16 0.5 0.05 60 20 100 42"
+    assert lines[1] == "diag 1 100 1e-12"
+    assert "big ok" in lines
