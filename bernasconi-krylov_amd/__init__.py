@@ -56,6 +56,10 @@ class EvolveParams(C.Structure):
                 ("measure_every", C.c_int)]
 
 
+class AdjointParams(C.Structure):
+    _fields_ = [("ev", EvolveParams), ("checkpoint_every", C.c_int)]
+
+
 class Series(C.Structure):
     _fields_ = [("p_less", C.POINTER(C.c_double)), ("dh", C.POINTER(C.c_double)),
                 ("norm", C.POINTER(C.c_double)), ("capacity", C.c_int64), ("count", C.c_int64),
@@ -118,6 +122,12 @@ _SIGS = {
     "bk_state_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bk_evolve": (C.c_int, [C.c_void_p, C.POINTER(EvolveParams), C.POINTER(Series)]),
+    "bk_adjoint_forward": (C.c_int, [C.c_void_p, C.POINTER(AdjointParams), C.POINTER(C.c_double)]),
+    "bk_adjoint_backward": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
+    "bk_adjoint_grad": (C.c_int, [C.c_void_p, C.POINTER(AdjointParams), C.c_double, C.c_double,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
     "bk_observables": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.POINTER(C.c_double)]),
     "bk_count_below": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_uint64)]),
     "bk_sample": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
@@ -300,6 +310,32 @@ class Engine:
         k = s.count
         return {"p_less": pl[:k], "dh": dh[:k], "norm": nm[:k], "e0": s.e0,
                 "p_blind": s.p_blind, "_keep": gs}
+
+    # ---- adjoint gradient ----
+    def adjoint_forward(self, n: int, start_bits: int, g_steps: Sequence[float], dt: float,
+                        order: int = BK_STRANG, precision: int = BK_C64, e0: int = -1,
+                        checkpoint_every: int = 64) -> tuple:
+        """Forward pass that records the checkpoint tape.  Returns (P_<, <d_H>, norm)
+        of the final state."""
+        gs = np.ascontiguousarray(g_steps, dtype=np.float64)
+        steps = int(gs.shape[0])
+        ev = EvolveParams(n=n, precision=precision, order=order, dt=dt, steps=steps, g=0.0,
+                          g_steps=_dp(gs), start_bits=start_bits, e0=e0, init_from_start=1,
+                          measure_every=0)
+        p = AdjointParams(ev=ev, checkpoint_every=checkpoint_every)
+        obs = np.zeros(3)
+        _check(lib().bk_adjoint_forward(self._h, C.byref(p), _dp(obs)))
+        self._adj_steps = steps
+        return float(obs[0]), float(obs[1]), float(obs[2])
+
+    def adjoint_backward(self, w_pless: float = 1.0, w_dh: float = 0.0):
+        """dL/dg_k per step for L = w_pless P_<(T) + w_dh <d_H>(T) over the tape of the
+        last adjoint_forward.  Returns (grad, drift)."""
+        grad = np.zeros(getattr(self, "_adj_steps", 0))
+        drift = C.c_double()
+        _check(lib().bk_adjoint_backward(self._h, float(w_pless), float(w_dh), _dp(grad),
+                                         C.byref(drift)))
+        return grad, drift.value
 
     def observables(self, s0: int, e0: int):
         out = np.zeros(3)

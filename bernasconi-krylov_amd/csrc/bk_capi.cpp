@@ -161,6 +161,21 @@ struct bk_ctx {
     int ecache_prec = 0;
     bool timing = false;
     EventTimer timer;
+    // bumped by every call that changes the held state; an adjoint tape is
+    // valid only for the generation it was recorded at
+    uint64_t state_gen = 0;
+    // adjoint tape: the forward run's parameters and its checkpoint states
+    struct Tape {
+        bool valid = false;
+        uint64_t gen = 0;
+        int n = 0, precision = 0, steps = 0, every = 0, nckpt = 0;
+        double dt = 0.0, theta = 0.0;
+        uint64_t s0 = 0;
+        uint32_t e0 = 0;
+        std::vector<double> g;  // field of each step
+        DevBuf ckpt;            // state after every-th step, 2*every-th, ... (< steps)
+    } tape;
+    DevBuf adj_phi, adj_chi, adj_part, adj_grad, adj_drift;
 };
 
 namespace {
@@ -234,6 +249,7 @@ int ensure_state(bk_ctx* c, int n, int precision)
     c->precision = precision;
     c->has_state = true;
     c->theta_pre = 0.0;
+    ++c->state_gen;
     if (!c->tuning_pinned) c->tuning = bk::default_tuning(n, precision);
     c->tuning.timer = c->timing ? &c->timer : nullptr;
     HIPCALL(c->partials.ensure(bk::partials_bytes(c->tuning.grid)));
@@ -308,6 +324,12 @@ void bk_ctx_destroy(bk_ctx* c)
     c->blas_partials.release();
     c->blas_out.release();
     c->ecache.release();
+    c->tape.ckpt.release();
+    c->adj_phi.release();
+    c->adj_chi.release();
+    c->adj_part.release();
+    c->adj_grad.release();
+    c->adj_drift.release();
     for (auto& kv : c->phase_tabs) kv.second.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -454,6 +476,18 @@ static int prepare_opening(bk_ctx* c, int order, double dt, bool fresh_start, ui
     return BK_OK;
 }
 
+// the uint16 E(s) cache of the held state's n, in the LOW-sweep layout of its precision
+static int ensure_ecache(bk_ctx* c)
+{
+    if (c->ecache_n != c->n || c->ecache_prec != c->precision) {
+        HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
+        HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream));
+        c->ecache_n = c->n;
+        c->ecache_prec = c->precision;
+    }
+    return BK_OK;
+}
+
 // Enqueue one Trotter step of the held state: the rotations, then the phase
 // exp(-i dt E/n) -- for Lie the next step's opening phase, for Strang this
 // step's closing half plus the next step's opening half -- so theta_pre is
@@ -473,14 +507,10 @@ static int enqueue_step(bk_ctx* c, double dt, double g, uint64_t s0, uint32_t e0
     sp.s0 = static_cast<uint32_t>(s0);
     sp.e0 = e0;
     if (c->energy_mode == BK_ENERGY_CACHE16 && c->n >= 13) {
-        if (c->ecache_n != c->n || c->ecache_prec != c->precision) {
-            HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
-            HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream));
-            c->ecache_n = c->n;
-            c->ecache_prec = c->precision;
-        }
+        if ((rc = ensure_ecache(c))) return rc;
         sp.ecache = c->ecache.as<uint16_t>();
     }
+    ++c->state_gen;
     void* alt = nullptr;
     if (c->tuning.pingpong && c->n >= 13) {
         HIPCALL(c->psi_alt.ensure(amp_bytes(c->precision) << c->n));
@@ -492,17 +522,24 @@ static int enqueue_step(bk_ctx* c, double dt, double g, uint64_t s0, uint32_t e0
     return BK_OK;
 }
 
-int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
+static int validate_evolve(const bk_evolve_params* p)
 {
-    int rc = check_ctx(c);
-    if (rc) return rc;
     if (!p) return fail(BK_E_INVALID, "null params");
     if (p->n < 1 || p->n > BK_MAX_N) return fail(BK_E_INVALID, "n=%d outside [1, %d]", p->n, BK_MAX_N);
+    if (p->precision != BK_C64 && p->precision != BK_C128) return fail(BK_E_INVALID, "precision=%d", p->precision);
     if (p->order != BK_LIE && p->order != BK_STRANG) return fail(BK_E_INVALID, "order=%d", p->order);
     if (!is_fin(p->dt) || !is_fin(p->g) || p->steps < 0) return fail(BK_E_INVALID, "non-finite dt/g or steps < 0");
     if (p->g_steps)
         for (int k = 0; k < p->steps; ++k)
             if (!is_fin(p->g_steps[k])) return fail(BK_E_INVALID, "non-finite g_steps[%d]", k);
+    return BK_OK;
+}
+
+int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = validate_evolve(p))) return rc;
     const uint64_t mask = (1ull << p->n) - 1ull;
     const uint64_t s0 = p->start_bits & mask;
     const int64_t e0 = p->e0 >= 0 ? p->e0 : bk::host_energy(p->n, s0);
@@ -558,6 +595,136 @@ int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
         HIPCALL(hipStreamSynchronize(c->stream));
     }
     return BK_OK;
+}
+
+// ---- adjoint gradient ----
+// Stored states carry P(theta) = exp(-i theta E/n), theta = the opening phase
+// (phi_k = P(theta) psi_k), so one step is phi_k = P(dt) X(g_k dt) phi_{k-1}
+// whatever the order.  The reverse pass holds phi'_k = P(-dt) phi_k and the
+// adjoint chi'_k = P(-dt) P(theta) lambda_k (lambda_T = W psi_T, W the
+// objective's diagonal weight): then phi'_{k-1} = P(-dt) X(-g_k dt) phi'_k,
+// the same for chi', and dL/dg_k = 2 Re <lambda_k| dU_k/dg |psi_{k-1}>
+// = -2 dt Im <chi'_k| sum_q X_q |phi'_k> (dX/dg = i dt sum_q X_q X).
+int bk_adjoint_forward(bk_ctx* c, const bk_adjoint_params* p, double* obs3)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!p) return fail(BK_E_INVALID, "null params");
+    const bk_evolve_params& ev = p->ev;
+    if ((rc = validate_evolve(&ev))) return rc;
+    if (ev.steps < 1) return fail(BK_E_INVALID, "adjoint needs steps >= 1");
+    if (!ev.init_from_start) return fail(BK_E_INVALID, "adjoint runs start from |start_bits> (init_from_start = 1)");
+    if (p->checkpoint_every < 1) return fail(BK_E_INVALID, "checkpoint_every=%d < 1", p->checkpoint_every);
+    c->tape.valid = false;
+    const uint64_t s0 = ev.start_bits & ((1ull << ev.n) - 1ull);
+    const int64_t e0 = ev.e0 >= 0 ? ev.e0 : bk::host_energy(ev.n, s0);
+    const uint32_t e0u = static_cast<uint32_t>(std::max<int64_t>(0, std::min<int64_t>(e0, 0xFFFFFFFFll)));
+    if ((rc = bk_state_init(c, ev.n, ev.precision, s0))) return rc;
+    const size_t bytes = amp_bytes(ev.precision) << ev.n;
+    const int K = p->checkpoint_every;
+    const int nck = (ev.steps - 1) / K;
+    if (nck > 0) {
+        hipError_t e = c->tape.ckpt.ensure(bytes * static_cast<size_t>(nck));
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? BK_E_NOMEM : BK_E_HIP, "adjoint tape of %d states (%.2f GB): %s", nck,
+                        1e-9 * static_cast<double>(bytes) * nck, hipGetErrorString(e));
+    }
+    HIPCALL(c->series.ensure(3 * sizeof(double)));
+    if ((rc = prepare_opening(c, ev.order, ev.dt, true, s0, e0u, nullptr))) return rc;
+    char* tape = static_cast<char*>(c->tape.ckpt.p);
+    for (int k = 0; k < ev.steps; ++k) {
+        const double g = ev.g_steps ? ev.g_steps[k] : ev.g;
+        if ((rc = enqueue_step(c, ev.dt, g, s0, e0u, k + 1 == ev.steps ? c->series.as<double>() : nullptr))) return rc;
+        if ((k + 1) % K == 0 && k + 1 < ev.steps)
+            HIPCALL(hipMemcpyAsync(tape + bytes * static_cast<size_t>((k + 1) / K - 1), c->psi.p, bytes,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    }
+    double h[3];
+    HIPCALL(hipMemcpyAsync(h, c->series.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    if (obs3) std::memcpy(obs3, h, sizeof(h));
+    auto& t = c->tape;
+    t.n = ev.n;
+    t.precision = ev.precision;
+    t.steps = ev.steps;
+    t.every = K;
+    t.nckpt = nck;
+    t.dt = ev.dt;
+    t.theta = c->theta_pre;
+    t.s0 = s0;
+    t.e0 = e0u;
+    t.g.resize(static_cast<size_t>(ev.steps));
+    for (int k = 0; k < ev.steps; ++k) t.g[static_cast<size_t>(k)] = ev.g_steps ? ev.g_steps[k] : ev.g;
+    t.gen = c->state_gen;
+    t.valid = true;
+    return BK_OK;
+}
+
+int bk_adjoint_backward(bk_ctx* c, double w_pless, double w_dh, double* grad_steps, double* drift)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!grad_steps || !is_fin(w_pless) || !is_fin(w_dh)) return fail(BK_E_INVALID, "bk_adjoint_backward: bad argument");
+    auto& t = c->tape;
+    if (!t.valid || !c->has_state || t.gen != c->state_gen || c->n != t.n || c->precision != t.precision)
+        return fail(BK_E_NOSTATE, "no adjoint tape for the held state (run bk_adjoint_forward first)");
+    const int n = t.n, prec = t.precision;
+    const size_t bytes = amp_bytes(prec) << n;
+    const int low_tb = bk::default_tuning(n, prec).tile_bits;
+    HIPCALL(c->adj_phi.ensure(bytes));
+    HIPCALL(c->adj_chi.ensure(bytes));
+    HIPCALL(c->adj_grad.ensure(sizeof(double) * static_cast<size_t>(t.steps)));
+    HIPCALL(c->adj_drift.ensure(sizeof(double) * static_cast<size_t>(std::max(t.nckpt, 1))));
+    HIPCALL(c->adj_part.ensure(sizeof(double) * std::max<size_t>(4096, bk::reverse_partials_count(n, prec, c->tuning))));
+    bk::StepSpec sp{};
+    sp.n = n;
+    sp.precision = prec;
+    sp.dtp = -t.dt;
+    if (n >= low_tb) {
+        if ((rc = ensure_ecache(c))) return rc;
+        sp.ecache = c->ecache.as<uint16_t>();
+    }
+    const void* tab_init = nullptr;
+    const void* tab_rev = nullptr;
+    // phi'_T = P(-dt) P(theta_tape - theta_now) * held state
+    if ((rc = get_phase_tab(c, n, prec, t.theta - c->theta_pre - t.dt, &tab_init))) return rc;
+    if ((rc = get_phase_tab(c, n, prec, -t.dt, &tab_rev))) return rc;
+    sp.phase_tab = tab_rev;
+    HIPCALL(bk::launch_adj_init(c->psi.p, c->adj_phi.p, c->adj_chi.p, n, prec, tab_init, static_cast<uint32_t>(t.s0), t.e0,
+                                w_pless, w_dh, c->stream));
+    const char* tape = static_cast<const char*>(t.ckpt.p);
+    double* dg = c->adj_grad.as<double>();
+    double* dd = c->adj_drift.as<double>();
+    double* part = c->adj_part.as<double>();
+    bk::SweepTuning tun = c->tuning;
+    tun.timer = c->timing ? &c->timer : nullptr;
+    for (int k = t.steps; k >= 1; --k) {
+        if (k < t.steps && k % t.every == 0) {
+            const int i = k / t.every - 1;
+            HIPCALL(bk::launch_phase_from(c->adj_phi.p, tape + bytes * static_cast<size_t>(i), n, prec, tab_rev, part,
+                                          dd + i, c->stream));
+        }
+        sp.gdt = -t.g[static_cast<size_t>(k - 1)] * t.dt;
+        HIPCALL(bk::launch_reverse_step(c->adj_phi.p, c->adj_chi.p, sp, tun, part, dg + (k - 1), c->stream));
+    }
+    std::vector<double> raw(static_cast<size_t>(t.steps));
+    std::vector<double> dr(static_cast<size_t>(std::max(t.nckpt, 1)), 0.0);
+    HIPCALL(hipMemcpyAsync(raw.data(), dg, raw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (t.nckpt > 0)
+        HIPCALL(hipMemcpyAsync(dr.data(), dd, static_cast<size_t>(t.nckpt) * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < t.steps; ++k) grad_steps[k] = -2.0 * t.dt * raw[static_cast<size_t>(k)];
+    if (drift) *drift = t.nckpt > 0 ? *std::max_element(dr.begin(), dr.begin() + t.nckpt) : 0.0;
+    return BK_OK;
+}
+
+int bk_adjoint_grad(bk_ctx* c, const bk_adjoint_params* p, double w_pless, double w_dh, double* obs3,
+                    double* grad_steps, double* drift)
+{
+    int rc = bk_adjoint_forward(c, p, obs3);
+    if (rc) return rc;
+    return bk_adjoint_backward(c, w_pless, w_dh, grad_steps, drift);
 }
 
 int bk_sample(bk_ctx* c, const double* randnums, int64_t shots, int64_t* out_idx)
@@ -685,6 +852,7 @@ int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, dou
     if (c->precision != BK_C128) return fail(BK_E_INVALID, "Krylov propagation needs a complex128 state");
     if (m < 2 || !is_fin(g) || !is_fin(tau) || !(tau_restart > 0.0)) return fail(BK_E_INVALID, "bad argument");
     if ((rc = flush_phase(c))) return rc;
+    ++c->state_gen;
     const int n = c->n;
     const int64_t dim = 1ll << n;
     DevBuf V, w, coef;

@@ -91,9 +91,9 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-// Deterministic block reduction of NV doubles (256 threads = 4 waves).
-// 'scratch' must hold 4*NV doubles and be free for use.
-template <int NV>
+// Deterministic block reduction of NV doubles over NW waves (pairwise tree
+// over the waves).  'scratch' must hold NW*NV doubles and be free for use.
+template <int NV, int NW = 4>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* scratch, double* out)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -105,9 +105,16 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* scra
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i)
-            out[i] = (scratch[0 * NV + i] + scratch[1 * NV + i]) +
-                     (scratch[2 * NV + i] + scratch[3 * NV + i]);
+        for (int i = 0; i < NV; ++i) {
+            double t[NW];
+#pragma unroll
+            for (int w = 0; w < NW; ++w) t[w] = scratch[w * NV + i];
+#pragma unroll
+            for (int h = 1; h < NW; h <<= 1)
+#pragma unroll
+                for (int w = 0; w + h < NW; w += 2 * h) t[w] += t[w + h];
+            out[i] = t[0];
+        }
     }
 }
 
@@ -121,14 +128,21 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // ---------------------------------------------------------------------------
 // Tiled sweep kernel
 // ---------------------------------------------------------------------------
-enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16 };
+// F_GRAD (two-state sweeps only): before each qubit's rotation, accumulate
+// Im sum_s conj(chi_s) phi_{s ^ 2^q} over the tile (chi = second state); the
+// sum over all rotated qubits is Im <chi| sum_q X_q |phi>, unchanged by the
+// rotations because both states receive the same ones and every X_q commutes
+// with them.  One double per workgroup.
+enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32 };
 
 struct SweepArgs {
     uint64_t ntiles;
     const void* tab;   // phase table (V per energy) for F_PHASE
-    double* partials;  // gridDim.x * 3 doubles for F_MEASURE
+    double* partials;  // gridDim.x * 3 doubles for F_MEASURE, gridDim.x for F_GRAD
     const uint16_t* ecache;  // F_ECACHE: E(s) per amplitude (exact, < 2^16)
     void* dst;               // output state (null: in place)
+    void* psi2;              // two-state sweeps: the second state (same tiles, same rotations)
+    void* dst2;              // its output (null: in place)
     double bc, bs;     // butterfly coefficients (F_TAN: bs = tan, bc unused)
     double scale;      // overall factor applied with the phase (c^n in F_TAN form)
     int n;
@@ -152,14 +166,29 @@ struct LowPlan {
     static constexpr Plan plan = make_low_plan(TB_, RB_);
 };
 
-template <typename V, class PH, int FLAGS, int SW>
+// NS = 2 moves two states through the same tiles and rotations (the reverse
+// pass of the adjoint gradient); they share the LDS buffer one after the other.
+template <typename V, class PH, int FLAGS, int SW, int NS = 1>
 __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const SweepArgs a)
 {
     using R = typename RealOf<V>::T;
     constexpr Plan P = PH::plan;
     constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R, TT = P.TT();
+    constexpr int NW = (1 << TT) / 64;  // waves per workgroup
     static_assert(!P.low || TT == kThreadBits, "the LOW sweep runs 256 threads");
+    static_assert(NS == 1 || NS == 2, "one or two states");
+    static_assert(NS == 1 || (FLAGS & (F_MEASURE | F_PREFETCH | F_TAN)) == 0,
+                  "two-state sweeps use the (c, s) butterfly without measurement or prefetch");
+    static_assert((FLAGS & F_GRAD) == 0 || NS == 2, "the gradient pairs two states");
     constexpr bool LOW = P.low;
+    V* sp[NS];
+    V* dp[NS];
+    sp[0] = psi;
+    dp[0] = a.dst ? static_cast<V*>(a.dst) : psi;
+    if constexpr (NS == 2) {
+        sp[1] = static_cast<V*>(a.psi2);
+        dp[1] = a.dst2 ? static_cast<V*>(a.dst2) : sp[1];
+    }
     constexpr uint32_t ROWMASK = (1u << RR) - 1u;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* lds = reinterpret_cast<V*>(smem_raw);
@@ -173,7 +202,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         for (int b = 0; b < TT; ++b) u |= ((t >> b) & 1u) << P.ph[p].thr[b];
         tu[p] = u;
     }
-    // global offset of a local index (HIGH: rows stay, active bits move to h)
+    // offset of a local index from the tile base (HIGH: rows stay, active bits
+    // move to h and h2)
     auto gl = [&](uint32_t u) -> uint64_t {
         if constexpr (LOW) return u;
         else {
@@ -184,8 +214,27 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     };
     const uint64_t tg0 = gl(tu[0]);
     const uint64_t tgL = gl(tu[NP - 1]);
+    // Two-state HIGH sweeps address register j as the sum of the (uniform)
+    // strides of its register bits, rebuilt per tile on the scalar unit: kept
+    // across the loop, the 2 x NR 64-bit offsets would not fit beside 2 x NR
+    // amplitudes in the register file.
+    constexpr bool STRIDED = !LOW && NS == 2;
+    uint64_t lstr[RB], sstr[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+        lstr[b] = gl(1u << P.ph[0].reg[b]);
+        sstr[b] = gl(1u << P.ph[NP - 1].reg[b]);
+    }
+    auto reg_off = [&](const uint64_t(&str)[RB], int j) -> uint64_t {
+        uint64_t o = 0;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+            if ((j >> b) & 1) o += str[b];
+        return o;
+    };
     const R bc = static_cast<R>(a.bc), bs = static_cast<R>(a.bs);
     double acc[3] = {0.0, 0.0, 0.0};
+    double gacc[1] = {0.0};
     constexpr bool PF = (FLAGS & F_PREFETCH) != 0;
     // energy cache words: the cache is stored tile by tile in the last phase's
     // register layout (thread t's NR values contiguous), two uint16 per word
@@ -203,12 +252,30 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         }
     };
     // phase-0 layout loads of one tile (amplitudes and, with the cache, E(s))
-    auto load = [&](uint64_t base, V(&xv)[NR], uint32_t(&ev)[NEC]) {
-        const V* tp = psi + base;
+    auto load = [&](uint64_t base, V(&xv)[NS][NR], uint32_t(&ev)[NEC]) {
+        if constexpr (STRIDED) {
+            uint64_t str[RB];
 #pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            const uint32_t pj = reg_pattern(P.ph[0], RB, j);
-            xv[j] = tp[tg0 + gl(pj)];
+            for (int b = 0; b < RB; ++b) {
+                str[b] = lstr[b];
+                asm volatile("" : "+v"(str[b]));  // opaque per tile: no hoisting
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const V* tp = sp[s] + base + tg0;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) xv[s][j] = tp[reg_off(str, j)];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const V* tp = sp[s] + base;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    const uint32_t pj = reg_pattern(P.ph[0], RB, j);
+                    xv[s][j] = tp[tg0 + gl(pj)];
+                }
+            }
         }
         if constexpr ((FLAGS & F_ECACHE) != 0) {
             const uint4* ep = reinterpret_cast<const uint4*>(a.ecache + base + static_cast<uint64_t>(t) * NR);
@@ -223,42 +290,85 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         }
     };
 
-    V x[NR];
+    V x[NS][NR];
     uint32_t ec[NEC];
+    // Two-state sweeps load each tile at the top of its iteration.  Loading
+    // the next tile at the bottom (as the one-state sweeps do) makes the loop
+    // header merge two loads in a phi, which the compiler rewrites as a load
+    // of a phi of pointers: 2 x NR 64-bit addresses carried around the loop,
+    // more than the register file holds beside 2 x NR amplitudes.
+    constexpr bool TOPLOAD = NS == 2;
     uint64_t tile = blockIdx.x;
-    if (tile < a.ntiles) load(tile_base(tile), x, ec);
+    if constexpr (!TOPLOAD) {
+        if (tile < a.ntiles) load(tile_base(tile), x, ec);
+    }
     for (; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t base = tile_base(tile);
         const uint64_t nxt = tile + gridDim.x;
+        if constexpr (TOPLOAD) load(base, x, ec);
         // software pipelining: the next tile's loads are issued before this
         // tile's stores, so waiting for them never waits on the stores
-        V xn[PF ? NR : 1];
+        V xn[NS][PF ? NR : 1];
         uint32_t ecn[PF ? NEC : 1];
         if constexpr (PF) {
             if (nxt < a.ntiles) load(tile_base(nxt), xn, ecn);
         }
+        R tg = R(0);
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             if (p > 0) {
-                __syncthreads();
+                if constexpr (NS == 2) {
+                    // swz is linear over XOR and the thread and register bits are
+                    // disjoint: swz(tu | pat) = swz(tu) ^ swz(pat).  The thread
+                    // part is made opaque per tile so the 2 x NR addresses of
+                    // each phase are one XOR each instead of registers held
+                    // across the loop.
+                    uint32_t wb = swz<SW>(tu[p - 1]), rb = swz<SW>(tu[p]);
+                    asm volatile("" : "+v"(wb), "+v"(rb));
 #pragma unroll
-                for (int j = 0; j < NR; ++j)
-                    lds[swz<SW>(tu[p - 1] | reg_pattern(P.ph[p - 1], RB, j))] = x[j];
-                __syncthreads();
+                    for (int s = 0; s < NS; ++s) {
+                        __syncthreads();
 #pragma unroll
-                for (int j = 0; j < NR; ++j) x[j] = lds[swz<SW>(tu[p] | reg_pattern(P.ph[p], RB, j))];
+                        for (int j = 0; j < NR; ++j) lds[wb ^ swz<SW>(reg_pattern(P.ph[p - 1], RB, j))] = x[s][j];
+                        __syncthreads();
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) x[s][j] = lds[rb ^ swz<SW>(reg_pattern(P.ph[p], RB, j))];
+                    }
+                } else {
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < NR; ++j)
+                        lds[swz<SW>(tu[p - 1] | reg_pattern(P.ph[p - 1], RB, j))] = x[0][j];
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) x[0][j] = lds[swz<SW>(tu[p] | reg_pattern(P.ph[p], RB, j))];
+                }
             }
 #pragma unroll
             for (int b = 0; b < RB; ++b) {
                 if (!((P.ph[p].act >> b) & 1u)) continue;
+                if constexpr ((FLAGS & F_GRAD) != 0) {
+                    // Im conj(chi_j) phi_j' + Im conj(chi_j') phi_j over the pairs of bit b
 #pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    if ((j >> b) & 1) continue;
-                    if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[j], x[j | (1 << b)], bs);
-                    else bfly_cs<V, R>(x[j], x[j | (1 << b)], bc, bs);
+                    for (int j = 0; j < NR; ++j) {
+                        if ((j >> b) & 1) continue;
+                        const V c0 = x[1][j], c1 = x[1][j | (1 << b)];
+                        const V f0 = x[0][j], f1 = x[0][j | (1 << b)];
+                        tg += fma(c0.x, f1.y, -c0.y * f1.x);
+                        tg += fma(c1.x, f0.y, -c1.y * f0.x);
+                    }
                 }
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) {
+                        if ((j >> b) & 1) continue;
+                        if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[s][j], x[s][j | (1 << b)], bs);
+                        else bfly_cs<V, R>(x[s][j], x[s][j | (1 << b)], bc, bs);
+                    }
             }
         }
+        if constexpr ((FLAGS & F_GRAD) != 0) gacc[0] += static_cast<double>(tg);
         // after the last rotation of the step: scale (c^n in tan form) and the
         // next step's phase exp(-i dt E/n), then P_< / <d_H> / norm of the
         // stepped state (|psi|^2 does not see the phase)
@@ -276,14 +386,17 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                 if constexpr ((FLAGS & F_PHASE) != 0) {
                     const V ph = tab[E >> 2];  // E = 4m + (n/2 mod 4): one entry per m
                     const R pr = ph.x * sc, pi = ph.y * sc;
-                    const R xr = x[j].x, xi = x[j].y;
-                    x[j].x = xr * pr - xi * pi;
-                    x[j].y = xr * pi + xi * pr;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const R xr = x[s][j].x, xi = x[s][j].y;
+                        x[s][j].x = xr * pr - xi * pi;
+                        x[s][j].y = xr * pi + xi * pr;
+                    }
                 }
                 if constexpr ((FLAGS & F_MEASURE) != 0) {
                     // per-tile partials in the state's precision (a thread's
                     // NR terms), folded into FP64 once per tile
-                    const R pr = fma(x[j].x, x[j].x, x[j].y * x[j].y);
+                    const R pr = fma(x[0][j].x, x[0][j].x, x[0][j].y * x[0][j].y);
                     tacc[2] += pr;
                     tacc[0] += (E < a.e0) ? pr : R(0);
                     tacc[1] = fma(pr, static_cast<R>(__popc(s ^ a.s0)), tacc[1]);
@@ -294,27 +407,49 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                 for (int i = 0; i < 3; ++i) acc[i] += static_cast<double>(tacc[i]);
             }
         }
-        V* op = (a.dst ? static_cast<V*>(a.dst) : psi) + base;
+        if constexpr (STRIDED) {
+            uint64_t str[RB];
 #pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            const uint32_t pj = reg_pattern(P.ph[NP - 1], RB, j);
-            op[tgL + gl(pj)] = x[j];
+            for (int b = 0; b < RB; ++b) {
+                str[b] = sstr[b];
+                asm volatile("" : "+v"(str[b]));
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                V* op = dp[s] + base + tgL;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) op[reg_off(str, j)] = x[s][j];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                V* op = dp[s] + base;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    const uint32_t pj = reg_pattern(P.ph[NP - 1], RB, j);
+                    op[tgL + gl(pj)] = x[s][j];
+                }
+            }
         }
         if (nxt < a.ntiles) {
             if constexpr (PF) {
 #pragma unroll
-                for (int j = 0; j < NR; ++j) x[j] = xn[j];
+                for (int j = 0; j < NR; ++j) x[0][j] = xn[0][j];
                 if constexpr ((FLAGS & F_ECACHE) != 0)
 #pragma unroll
                     for (int j = 0; j < NEC; ++j) ec[j] = ecn[j];
-            } else {
+            } else if constexpr (!TOPLOAD) {
                 load(tile_base(nxt), x, ec);
             }
         }
     }
     if constexpr ((FLAGS & F_MEASURE) != 0) {
         __syncthreads();
-        block_reduce_store<3>(acc, reinterpret_cast<double*>(smem_raw), a.partials + 3 * blockIdx.x);
+        block_reduce_store<3, NW>(acc, reinterpret_cast<double*>(smem_raw), a.partials + 3 * blockIdx.x);
+    }
+    if constexpr ((FLAGS & F_GRAD) != 0) {
+        __syncthreads();
+        block_reduce_store<1, NW>(gacc, reinterpret_cast<double*>(smem_raw), a.partials + blockIdx.x);
     }
 }
 
@@ -450,6 +585,124 @@ __global__ __launch_bounds__(kThreads) void k_finish_u64(const unsigned long lon
     if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) *out = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
+}
+
+// sum 'count' doubles in a fixed order into *out
+__global__ __launch_bounds__(kThreads) void k_finish_sum(const double* __restrict__ partials, int count,
+                                                        double* __restrict__ out)
+{
+    __shared__ double scratch[4];
+    double acc[1] = {0.0};
+    for (int i = threadIdx.x; i < count; i += blockDim.x) acc[0] += partials[i];
+    block_reduce_store<1>(acc, scratch, out);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint-pass kernels (two states: phi = the reversed forward state, chi =
+// the adjoint state)
+// ---------------------------------------------------------------------------
+
+// one qubit's rotation of both states (states below the tile size), with the
+// pair term Im conj(chi_s) phi_{s^q} + Im conj(chi_{s^q}) phi_s taken before it
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_qubit2(V* __restrict__ phi, V* __restrict__ chi, int q,
+                                                    uint64_t npairs, double bc_, double bs_,
+                                                    double* __restrict__ partials)
+{
+    using R = typename RealOf<V>::T;
+    __shared__ double scratch[4];
+    const R bc = static_cast<R>(bc_), bs = static_cast<R>(bs_);
+    const uint64_t st = 1ull << q;
+    double acc[1] = {0.0};
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < npairs;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t lo = ((i >> q) << (q + 1)) | (i & (st - 1));
+        V f0 = phi[lo], f1 = phi[lo | st], c0 = chi[lo], c1 = chi[lo | st];
+        acc[0] += static_cast<double>(fma(c0.x, f1.y, -c0.y * f1.x)) +
+                  static_cast<double>(fma(c1.x, f0.y, -c1.y * f0.x));
+        bfly_cs<V, R>(f0, f1, bc, bs);
+        bfly_cs<V, R>(c0, c1, bc, bs);
+        phi[lo] = f0;
+        phi[lo | st] = f1;
+        chi[lo] = c0;
+        chi[lo | st] = c1;
+    }
+    block_reduce_store<1>(acc, scratch, partials + blockIdx.x);
+}
+
+// both states times exp(-i dtp E/n) (the table entry of E)
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_phase2(V* __restrict__ phi, V* __restrict__ chi, uint64_t dim,
+                                                    const void* tab_, int n)
+{
+    using R = typename RealOf<V>::T;
+    const V* tab = static_cast<const V*>(tab_);
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const V ph = tab[labs_energy_u32(static_cast<uint32_t>(i), n) >> 2];
+        V f = phi[i], c = chi[i];
+        const R fr = f.x, cr = c.x;
+        f.x = fr * ph.x - f.y * ph.y;
+        f.y = fr * ph.y + f.y * ph.x;
+        c.x = cr * ph.x - c.y * ph.y;
+        c.y = cr * ph.y + c.y * ph.x;
+        phi[i] = f;
+        chi[i] = c;
+    }
+}
+
+// Start of the reverse pass: phi = P * src, chi = W * phi with the diagonal
+// objective weight W(s) = wp [E(s) < e0] + wd popcount(s ^ s0).
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_adj_init(const V* __restrict__ src, V* __restrict__ phi,
+                                                      V* __restrict__ chi, uint64_t dim, const void* tab_,
+                                                      int n, uint32_t s0, uint32_t e0, double wp, double wd)
+{
+    using R = typename RealOf<V>::T;
+    const V* tab = static_cast<const V*>(tab_);
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t s = static_cast<uint32_t>(i);
+        const uint32_t E = labs_energy_u32(s, n);
+        const V ph = tab[E >> 2];
+        const V x = src[i];
+        V f;
+        f.x = x.x * ph.x - x.y * ph.y;
+        f.y = x.x * ph.y + x.y * ph.x;
+        const R w = static_cast<R>((E < e0 ? wp : 0.0) + wd * static_cast<double>(__popc(s ^ s0)));
+        V c;
+        c.x = w * f.x;
+        c.y = w * f.y;
+        phi[i] = f;
+        chi[i] = c;
+    }
+}
+
+// Checkpoint reset: dst <- P * src, with sum |new - old|^2 per block (the
+// drift of the reversed state from the stored forward state).
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_phase_from(V* __restrict__ dst, const V* __restrict__ src,
+                                                        uint64_t dim, uint64_t chunk, const void* tab_, int n,
+                                                        double* __restrict__ partials)
+{
+    __shared__ double scratch[4];
+    const V* tab = static_cast<const V*>(tab_);
+    double acc[1] = {0.0};
+    const uint64_t beg = blockIdx.x * chunk;
+    const uint64_t end = (beg + chunk < dim) ? beg + chunk : dim;
+    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        const V ph = tab[labs_energy_u32(static_cast<uint32_t>(i), n) >> 2];
+        const V x = src[i];
+        V f;
+        f.x = x.x * ph.x - x.y * ph.y;
+        f.y = x.x * ph.y + x.y * ph.x;
+        const V o = dst[i];
+        const double dr = static_cast<double>(f.x) - static_cast<double>(o.x);
+        const double di = static_cast<double>(f.y) - static_cast<double>(o.y);
+        acc[0] += dr * dr + di * di;
+        dst[i] = f;
+    }
+    block_reduce_store<1>(acc, scratch, partials + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -676,6 +929,137 @@ hipError_t step_impl(V* psi, V* alt, int* in_alt, const StepSpec& sp, const Swee
     return hipSuccess;
 }
 
+// ---- reverse (adjoint) sweeps: two states, (c, s) butterfly, gradient terms ----
+template <typename V, int R>
+hipError_t launch_high_rev_r(V* phi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    using PH = HighPlan<C::HTB, C::HRB, R>;
+    auto k = k_sweep<V, PH, F_GRAD, C::SW, 2>;
+    const size_t lds = sizeof(V) << C::HTB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, phi, a);
+    return hipGetLastError();
+}
+
+template <typename V, int R = 3>
+hipError_t launch_high_rev(V* phi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (R > Cfg<V>::HTB - 1) {
+        return hipErrorInvalidValue;
+    } else {
+        if (rbits == R) return launch_high_rev_r<V, R>(phi, a, grid, st);
+        return launch_high_rev<V, R + 1>(phi, rbits, a, grid, st);
+    }
+}
+
+template <typename V>
+hipError_t launch_low_rev(V* phi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, F_PHASE | F_GRAD | F_ECACHE, C::SW, 2>;
+    const size_t lds = sizeof(V) << C::TB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, phi, a);
+    return hipGetLastError();
+}
+
+// One reverse step of both states: the rotations exp(i gdt X) per qubit
+// (sp.gdt = -g dt undoes a forward step of field g), the gradient terms of
+// the states as they enter, then exp(-i dtp E/n) from sp.phase_tab.
+// grad_slot <- Im <chi| sum_q X_q |phi> (FP64, deterministic order).
+template <typename V>
+hipError_t reverse_step_impl(V* phi, V* chi, const StepSpec& sp, const SweepTuning& tun, double* gpart,
+                             double* grad_slot, hipStream_t st)
+{
+    using C = Cfg<V>;
+    const int n = sp.n;
+    const double bc = std::cos(sp.gdt), bs = std::sin(sp.gdt);
+    const uint64_t dim = 1ull << n;
+    hipError_t e = hipSuccess;
+    if (n < C::TB) {
+        const uint64_t np = dim / 2;
+        const int qg = static_cast<int>(std::min<uint64_t>((np + kThreads - 1) / kThreads, 64));
+        for (int q = 0; q < n; ++q) {
+            hipLaunchKernelGGL(k_qubit2<V>, dim3(qg), dim3(kThreads), 0, st, phi, chi, q, np, bc, bs,
+                               gpart + static_cast<size_t>(q) * qg);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        const int pg = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 1024));
+        hipLaunchKernelGGL(k_phase2<V>, dim3(pg), dim3(kThreads), 0, st, phi, chi, dim, sp.phase_tab, n);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, gpart, n * qg, grad_slot);
+        return hipGetLastError();
+    }
+    SweepArgs a{};
+    a.tab = sp.phase_tab;
+    a.bc = bc;
+    a.bs = bs;
+    a.scale = 1.0;
+    a.n = n;
+    a.ecache = sp.ecache;
+    a.psi2 = chi;
+    a.ntiles = dim >> C::TB;
+    const double sweep_bytes = 4.0 * static_cast<double>(sizeof(V)) * static_cast<double>(dim);
+    int off = 0;
+    const std::vector<HighSweep> plan = plan_high_sweeps(n, C::TB, C::HTB, std::max(tun.row_bits, 3),
+                                                         tun.page_bits);
+    for (const HighSweep& hs : plan) {
+        SweepArgs b = a;
+        b.h = hs.h1;
+        b.a1 = hs.a1;
+        b.h2 = hs.h2;
+        b.lo_bits = hs.h1 - hs.R;
+        b.mid_bits = hs.h2 - (hs.h1 + hs.a1);
+        b.ntiles = dim >> C::HTB;
+        b.partials = gpart + off;
+        const int g = clamp_grid(b.ntiles, tun.grid);
+        if (tun.timer) tun.timer->begin(st);
+        e = launch_high_rev<V>(phi, hs.R, b, g, st);
+        if (e != hipSuccess) return e;
+        if (tun.timer) tun.timer->end(st, "adjoint_sweep_high", sweep_bytes);
+        off += g;
+    }
+    const int g1 = clamp_grid(a.ntiles, tun.grid);
+    a.partials = gpart + off;
+    if (tun.timer) tun.timer->begin(st);
+    if (!sp.ecache) return hipErrorInvalidValue;  // the reverse LOW sweep reads E(s) from the cache
+    e = launch_low_rev<V>(phi, a, g1, st);
+    if (e != hipSuccess) return e;
+    if (tun.timer) tun.timer->end(st, "adjoint_sweep_low", sweep_bytes);
+    off += g1;
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, gpart, off, grad_slot);
+    return hipGetLastError();
+}
+
+template <typename V>
+hipError_t adj_init_impl(const V* src, V* phi, V* chi, int n, const void* tab, uint32_t s0, uint32_t e0, double wp,
+                         double wd, hipStream_t st)
+{
+    const uint64_t dim = 1ull << n;
+    const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 16384));
+    hipLaunchKernelGGL(k_adj_init<V>, dim3(grid), dim3(kThreads), 0, st, src, phi, chi, dim, tab, n, s0, e0, wp,
+                       wd);
+    return hipGetLastError();
+}
+
+template <typename V>
+hipError_t phase_from_impl(V* dst, const V* src, int n, const void* tab, double* partials, double* out,
+                           hipStream_t st)
+{
+    const uint64_t dim = 1ull << n;
+    uint64_t chunk = 4096;
+    while ((dim + chunk - 1) / chunk > 4096) chunk <<= 1;
+    const int grid = static_cast<int>((dim + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_phase_from<V>, dim3(grid), dim3(kThreads), 0, st, dst, src, dim, chunk, tab, n, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, partials, grid, out);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 SweepTuning default_tuning(int n, int precision)
@@ -704,6 +1088,55 @@ hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepS
     if (spec.precision == kC128)
         return step_impl(static_cast<double2*>(psi), static_cast<double2*>(psi_alt), result_in_alt, spec, tun,
                          partials, stream);
+    return hipErrorInvalidValue;
+}
+
+size_t reverse_partials_count(int n, int precision, const SweepTuning& tun)
+{
+    const int tb = precision == kC64 ? Cfg<float2>::TB : Cfg<double2>::TB;
+    const int htb = precision == kC64 ? Cfg<float2>::HTB : Cfg<double2>::HTB;
+    const uint64_t dim = 1ull << n;
+    if (n < tb) return static_cast<size_t>(n) * 64;
+    size_t total = static_cast<size_t>(clamp_grid(dim >> tb, tun.grid));
+    for (size_t i = 0; i < plan_high_sweeps(n, tb, htb, std::max(tun.row_bits, 3), tun.page_bits).size(); ++i)
+        total += static_cast<size_t>(clamp_grid(dim >> htb, tun.grid));
+    return total;
+}
+
+hipError_t launch_reverse_step(void* phi, void* chi, const StepSpec& spec, const SweepTuning& tun, double* gpart,
+                               double* grad_slot, hipStream_t stream)
+{
+    if (spec.n < 1 || spec.n > 32) return hipErrorInvalidValue;
+    if (spec.precision == kC64)
+        return reverse_step_impl(static_cast<float2*>(phi), static_cast<float2*>(chi), spec, tun, gpart, grad_slot,
+                                 stream);
+    if (spec.precision == kC128)
+        return reverse_step_impl(static_cast<double2*>(phi), static_cast<double2*>(chi), spec, tun, gpart,
+                                 grad_slot, stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_init(const void* src, void* phi, void* chi, int n, int precision, const void* tab, uint32_t s0,
+                           uint32_t e0, double wp, double wd, hipStream_t stream)
+{
+    if (precision == kC64)
+        return adj_init_impl(static_cast<const float2*>(src), static_cast<float2*>(phi), static_cast<float2*>(chi), n,
+                             tab, s0, e0, wp, wd, stream);
+    if (precision == kC128)
+        return adj_init_impl(static_cast<const double2*>(src), static_cast<double2*>(phi), static_cast<double2*>(chi),
+                             n, tab, s0, e0, wp, wd, stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_phase_from(void* dst, const void* src, int n, int precision, const void* tab, double* partials,
+                             double* out, hipStream_t stream)
+{
+    if (precision == kC64)
+        return phase_from_impl(static_cast<float2*>(dst), static_cast<const float2*>(src), n, tab, partials, out,
+                               stream);
+    if (precision == kC128)
+        return phase_from_impl(static_cast<double2*>(dst), static_cast<const double2*>(src), n, tab, partials, out,
+                               stream);
     return hipErrorInvalidValue;
 }
 

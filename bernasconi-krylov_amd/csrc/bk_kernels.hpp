@@ -63,6 +63,22 @@ size_t partials_bytes(int grid);
 hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepSpec& spec,
                        const SweepTuning& tun, double* partials, hipStream_t stream);
 
+// ---- reverse pass of the adjoint gradient ----
+// One reverse step of phi (reversed forward state) and chi (adjoint state),
+// both in place: per-qubit rotation exp(i spec.gdt X) (spec.gdt = -g dt undoes
+// a step of field g), then exp(-i spec.dtp E/n) from spec.phase_tab (and
+// spec.ecache if set).  *grad_slot <- Im <chi| sum_q X_q |phi> of the states
+// as they enter.  gpart holds reverse_partials_count() doubles.
+hipError_t launch_reverse_step(void* phi, void* chi, const StepSpec& spec, const SweepTuning& tun, double* gpart,
+                               double* grad_slot, hipStream_t stream);
+size_t reverse_partials_count(int n, int precision, const SweepTuning& tun);
+// phi = tab[E] * src; chi = (wp [E < e0] + wd popcount(s ^ s0)) * phi
+hipError_t launch_adj_init(const void* src, void* phi, void* chi, int n, int precision, const void* tab, uint32_t s0,
+                           uint32_t e0, double wp, double wd, hipStream_t stream);
+// dst = tab[E] * src; *out = sum |new dst - old dst|^2 (partials: 4096 doubles)
+hipError_t launch_phase_from(void* dst, const void* src, int n, int precision, const void* tab, double* partials,
+                             double* out, hipStream_t stream);
+
 // Read-only measurement of the state into series_slot (3 doubles).
 hipError_t launch_measure(const void* psi, int n, int precision, uint32_t s0, uint32_t e0,
                           double* series_slot, double* partials, int grid, hipStream_t stream);

@@ -105,6 +105,33 @@ typedef struct {
 
 int bk_evolve(bk_ctx* ctx, const bk_evolve_params* p, bk_series* out);
 
+/* ---- adjoint gradient of the evolution (no reference counterpart: the
+ * reference has no differentiable path) ----
+ * Objective L = w_pless * P_<(T) + w_dh * <d_H>(T) of the state after
+ * ev.steps Trotter steps from |ev.start_bits> with field g_k in step k
+ * (ev.g_steps, or the constant ev.g).
+ * bk_adjoint_forward runs the steps like bk_evolve (init_from_start must be 1;
+ * measure_every is ignored), keeps the state after every checkpoint_every-th
+ * step on the device (the tape: floor((steps-1)/checkpoint_every) states) and
+ * writes {P_<, <d_H>, norm} of the final state to obs3 (may be NULL).  The
+ * final state stays held.
+ * bk_adjoint_backward steps both the state and the adjoint state backwards
+ * over the tape of the last bk_adjoint_forward on this context (BK_E_NOSTATE
+ * if the held state changed since): grad_steps[k] = dL/dg_k, k = 0..steps-1
+ * (host, caller-owned).  At every checkpoint the reversed state is replaced by
+ * the taped one; drift (may be NULL) receives the largest sum |reversed -
+ * taped|^2 seen there.  The held state is left unchanged. */
+typedef struct {
+    bk_evolve_params ev;
+    int checkpoint_every; /* >= 1 */
+} bk_adjoint_params;
+
+int bk_adjoint_forward(bk_ctx* ctx, const bk_adjoint_params* p, double* obs3);
+int bk_adjoint_backward(bk_ctx* ctx, double w_pless, double w_dh, double* grad_steps, double* drift);
+/* forward then backward */
+int bk_adjoint_grad(bk_ctx* ctx, const bk_adjoint_params* p, double w_pless, double w_dh, double* obs3,
+                    double* grad_steps, double* drift);
+
 /* P_<, <d_H>, norm of the held state (out[3]) */
 int bk_observables(bk_ctx* ctx, uint64_t s0, int64_t e0, double* out3);
 /* exact #{ s : E(s) < e0 } over 2^n states, on the device */

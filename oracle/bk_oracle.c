@@ -321,6 +321,115 @@ int bko_trotter(int n, uint64_t s0, int64_t e0, double g, const double *g_steps,
     return 0;
 }
 
+/* y = sum_q X_q x (every spin flipped once) */
+static void apply_sx(int n, const double *x, double *y)
+{
+    const int64_t dim = 1LL << n;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < dim; ++i) {
+        double yr = 0.0, yi = 0.0;
+        for (int q = 0; q < n; ++q) {
+            const int64_t j = i ^ (1LL << q);
+            yr += x[2 * j];
+            yi += x[2 * j + 1];
+        }
+        y[2 * i] = yr;
+        y[2 * i + 1] = yi;
+    }
+}
+
+/* U^+ of one step (order 1: U = X P(dt); order 2: U = P(dt/2) X P(dt/2)) */
+static void step_adjoint(int n, double *psi, const int32_t *en, double g, double dt, int order)
+{
+    if (order == 1) {
+        apply_mixer(n, psi, -g, dt);
+        apply_phase(n, psi, en, -dt);
+    } else {
+        apply_phase(n, psi, en, -0.5 * dt);
+        apply_mixer(n, psi, -g, dt);
+        apply_phase(n, psi, en, -0.5 * dt);
+    }
+}
+
+int bko_adjoint_grad(int n, uint64_t s0, int64_t e0, const double *g_steps, double dt,
+                     int steps, int order, double wp, double wd, double *grad, double *obs2,
+                     int threads)
+{
+    if (n < 1 || n > 30 || steps < 1 || (order != 1 && order != 2) || !g_steps || !grad) return -1;
+#ifdef _OPENMP
+    int saved = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    const int64_t dim = 1LL << n;
+    s0 &= mask_n(n);
+    int32_t *en = (int32_t *)malloc(sizeof(int32_t) * (size_t)dim);
+    double *psi = (double *)malloc(sizeof(double) * 2 * (size_t)dim);
+    double *lam = (double *)malloc(sizeof(double) * 2 * (size_t)dim);
+    double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)dim);
+    double *dpsi = (double *)malloc(sizeof(double) * 2 * (size_t)dim);
+    if (!en || !psi || !lam || !tmp || !dpsi) {
+        free(en);
+        free(psi);
+        free(lam);
+        free(tmp);
+        free(dpsi);
+        return -2;
+    }
+    bko_all_energies(n, en);
+    /* forward */
+    for (int64_t i = 0; i < 2 * dim; ++i) psi[i] = 0.0;
+    psi[2 * s0] = 1.0;
+    for (int k = 0; k < steps; ++k) {
+        if (order == 1) {
+            apply_phase(n, psi, en, dt);
+            apply_mixer(n, psi, g_steps[k], dt);
+        } else {
+            apply_phase(n, psi, en, 0.5 * dt);
+            apply_mixer(n, psi, g_steps[k], dt);
+            apply_phase(n, psi, en, 0.5 * dt);
+        }
+    }
+    if (obs2) observables_e(n, psi, en, s0, e0, &obs2[0], &obs2[1], NULL);
+    /* lambda_T = W psi_T */
+    for (int64_t i = 0; i < dim; ++i) {
+        const double w = (en[i] < e0 ? wp : 0.0) + wd * (double)__builtin_popcountll((uint64_t)i ^ s0);
+        lam[2 * i] = w * psi[2 * i];
+        lam[2 * i + 1] = w * psi[2 * i + 1];
+    }
+    for (int k = steps - 1; k >= 0; --k) {
+        const double g = g_steps[k];
+        step_adjoint(n, psi, en, g, dt, order); /* psi <- psi_{k} (state before step k) */
+        /* dpsi = dU/dg psi: the step's factors with i dt sum_q X_q after the mixer */
+        for (int64_t i = 0; i < 2 * dim; ++i) tmp[i] = psi[i];
+        if (order == 1) {
+            apply_phase(n, tmp, en, dt);
+            apply_mixer(n, tmp, g, dt);
+            apply_sx(n, tmp, dpsi);
+        } else {
+            apply_phase(n, tmp, en, 0.5 * dt);
+            apply_mixer(n, tmp, g, dt);
+            apply_sx(n, tmp, dpsi);
+            apply_phase(n, dpsi, en, 0.5 * dt);
+        }
+        /* grad = 2 Re <lam| i dt dpsi> = -2 dt Im <lam|dpsi> */
+        double im = 0.0;
+        for (int64_t i = 0; i < dim; ++i) im += lam[2 * i] * dpsi[2 * i + 1] - lam[2 * i + 1] * dpsi[2 * i];
+        grad[k] = -2.0 * dt * im;
+        step_adjoint(n, lam, en, g, dt, order);
+    }
+    free(en);
+    free(psi);
+    free(lam);
+    free(tmp);
+    free(dpsi);
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return 0;
+}
+
 int bko_trotter_bench(int n, uint64_t s0, double g, double dt, int steps, int order, int threads,
                       double *seconds)
 {

@@ -16,6 +16,7 @@
  *   bko_random_state /
  *   bko_steepest_descent       oracle/_ref (reference classical.cuh compiled here),
  *                              seed fixtures in tests/golden
+ *   bko_adjoint_grad           central finite differences of bko_trotter (tests)
  *   bko_trotter                probability_dynamics_n*.csv golden trajectories,
  *                              closed form <d_H>(dt) = N sin^2(g dt), mixer-only
  *                              closed form, dense exp(-iHt) via scipy at N <= 10
@@ -72,6 +73,18 @@ int bko_trotter(int n, uint64_t s0, int64_t e0, double g, const double *g_steps,
  * 'steps' Trotter steps (no observables).  Returns 0 and the step-loop seconds. */
 int bko_trotter_bench(int n, uint64_t s0, double g, double dt, int steps, int order,
                       int threads, double *seconds);
+/* Gradient of L = wp * P_<(T) + wd * <d_H>(T) with respect to the field of each
+ * step (g_steps[k], k = 0..steps-1) for the evolution of bko_trotter from |s0>
+ * (no reference counterpart: the reference has no differentiable path).
+ * Plain reverse mode in the unrotated frame: psi_T from a forward run,
+ * lambda_T = W psi_T; then for k from the last step down, psi_{k-1} = U_k^+ psi_k,
+ * grad[k] = 2 Re <lambda_k| dU_k/dg |psi_{k-1}> with dU/dg applied literally
+ * (the step's factors with i dt sum_q X_q inserted after the mixer), and
+ * lambda_{k-1} = U_k^+ lambda_k.  obs2 (may be NULL) = {P_<(T), <d_H>(T)}. */
+int bko_adjoint_grad(int n, uint64_t s0, int64_t e0, const double *g_steps, double dt,
+                     int steps, int order, double wp, double wd, double *grad, double *obs2,
+                     int threads);
+
 /* observables of one state */
 void bko_observables(int n, const double *psi, uint64_t s0, int64_t e0,
                      double *p_less, double *dh, double *norm);

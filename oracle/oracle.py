@@ -67,6 +67,9 @@ def lib():
         L.bko_trotter.argtypes = [C.c_int, _u64, _i64, C.c_double, _dp, C.c_double, C.c_int,
                                   C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_int]
         L.bko_trotter_bench.restype = C.c_int
+        L.bko_adjoint_grad.argtypes = [C.c_int, _u64, _i64, _dp, C.c_double, C.c_int, C.c_int,
+                                       C.c_double, C.c_double, _dp, _dp, C.c_int]
+        L.bko_adjoint_grad.restype = C.c_int
         L.bko_trotter_bench.argtypes = [C.c_int, _u64, C.c_double, C.c_double, C.c_int, C.c_int,
                                         C.c_int, _dp]
         L.bko_observables.argtypes = [C.c_int, _dp, _u64, _i64, _dp, _dp, _dp]
@@ -213,6 +216,21 @@ def trotter(n, s0, g, dt, steps, order=1, e0=None, g_steps=None, psi=None, threa
     if want_state:
         out["psi"] = state.view(np.complex128)
     return out
+
+
+def adjoint_grad(n, s0, g_steps, dt, order=2, e0=None, wp=1.0, wd=0.0, threads=0):
+    """dL/dg_k for L = wp*P_<(T) + wd*<d_H>(T) (FP64 reverse mode).  Returns
+    (grad per step, (P_<(T), <d_H>(T)))."""
+    if e0 is None:
+        e0 = energy(n, s0)
+    gs = np.ascontiguousarray(g_steps, dtype=np.float64)
+    grad = np.zeros(gs.shape[0])
+    obs = np.zeros(2)
+    rc = lib().bko_adjoint_grad(n, s0, e0, _p(gs), float(dt), int(gs.shape[0]), int(order), float(wp),
+                                float(wd), _p(grad), _p(obs), int(threads))
+    if rc != 0:
+        raise RuntimeError(f"bko_adjoint_grad failed rc={rc}")
+    return grad, (obs[0], obs[1])
 
 
 def trotter_bench(n, s0, g, dt, steps, order=2, threads=0):

@@ -1,0 +1,141 @@
+"""GPU parity of the adjoint gradient (bk_adjoint_forward / bk_adjoint_backward)
+against the oracle's FP64 reverse mode (bko_adjoint_grad, itself pinned by
+central differences and a closed form in test_oracle.py), against central
+differences of the GPU complex128 forward, and across checkpoint spacings."""
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(n_seg, seed, spread=0.05, base=0.35):
+    rng = np.random.default_rng(seed)
+    return base + spread * rng.uniform(-1.0, 1.0, n_seg)
+
+
+# n: 8 runs the per-qubit path, 12/13+ the tiled sweeps (c128 LOW tile = 12 bits,
+# c64 = 13), 15-16 add the high-qubit sweeps
+@pytest.mark.parametrize("n,order,w", [
+    (8, 2, (1.0, 0.0)), (8, 1, (0.0, 1.0)),
+    (12, 2, (1.0, 0.0)), (13, 1, (0.5, 0.25)),
+    (15, 2, (1.0, 0.0)), (16, 1, (0.0, 1.0)), (16, 2, (0.7, -0.2)),
+])
+def test_adjoint_c128_matches_oracle(bk, orc, engine, n, order, w):
+    _, s0, e0 = bk.make_trap(n, 3)
+    gs = np.repeat(_fields(6, n), 8)  # 6 segments x 8 steps
+    dt = 0.05
+    ref, (pl, dh) = orc.adjoint_grad(n, s0, gs, dt, order=order, e0=e0, wp=w[0], wd=w[1])
+    obs = engine.adjoint_forward(n, s0, gs, dt, order=order, precision=bk.BK_C128, e0=e0,
+                                 checkpoint_every=10)
+    assert obs[0] == pytest.approx(pl, abs=1e-12)
+    assert obs[1] == pytest.approx(dh, abs=1e-11)
+    assert obs[2] == pytest.approx(1.0, abs=1e-12)
+    grad, drift = engine.adjoint_backward(*w)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(grad - ref).max() <= 1e-10 * scale, (np.abs(grad - ref).max(), scale)
+    assert drift < 1e-20
+
+
+@pytest.mark.parametrize("n,order", [(14, 2), (16, 1), (18, 2)])
+def test_adjoint_c64_matches_oracle(bk, orc, engine, n, order):
+    """complex64 state, FP32 butterflies, FP64 reductions: within 5e-5 of the
+    largest gradient component (FP32 rounding accumulates along the reverse
+    pass; measured 2e-5 at n = 18 over 80 steps)."""
+    _, s0, e0 = bk.make_trap(n, 5)
+    gs = np.repeat(_fields(8, 100 + n), 10)
+    dt = 0.02
+    ref, (pl, _) = orc.adjoint_grad(n, s0, gs, dt, order=order, e0=e0, wp=1.0, wd=0.0)
+    obs = engine.adjoint_forward(n, s0, gs, dt, order=order, precision=bk.BK_C64, e0=e0,
+                                 checkpoint_every=16)
+    assert obs[0] == pytest.approx(pl, abs=1e-6)
+    grad, drift = engine.adjoint_backward(1.0, 0.0)
+    scale = np.abs(ref).max()
+    assert np.abs(grad - ref).max() <= 5e-5 * scale + 1e-9, (np.abs(grad - ref).max(), scale)
+    assert drift < 1e-9
+
+
+def test_adjoint_checkpoint_spacing_invariance(bk, engine):
+    """The gradient does not depend on where the tape resets the reversed state
+    (complex128: reversal is exact to rounding), and without checkpoints the
+    result is the same too."""
+    n = 14
+    _, s0, e0 = bk.make_trap(n, 9)
+    gs = np.repeat(_fields(5, 4), 12)
+    out = []
+    for K in (1, 7, 25, 60, 1000):
+        engine.adjoint_forward(n, s0, gs, 0.04, order=bk.BK_STRANG, precision=bk.BK_C128, e0=e0,
+                               checkpoint_every=K)
+        g, drift = engine.adjoint_backward(1.0, 0.3)
+        assert drift < 1e-20
+        out.append(g)
+    for g in out[1:]:
+        assert np.abs(g - out[0]).max() <= 1e-12 * max(1.0, np.abs(out[0]).max())
+
+
+def test_adjoint_matches_fd_of_gpu_forward(bk, engine):
+    """c64 adjoint against central differences (h = 1e-4) of complex128 GPU
+    forward runs, per segment, to 1e-4."""
+    n, sps, dt = 16, 20, 0.02
+    _, s0, e0 = bk.make_trap(n, 2)
+    g = _fields(8, 21)
+    engine.adjoint_forward(n, s0, np.repeat(g, sps), dt, order=bk.BK_STRANG, precision=bk.BK_C64,
+                           e0=e0, checkpoint_every=32)
+    grad, _ = engine.adjoint_backward(1.0, 0.0)
+    gseg = grad.reshape(len(g), sps).sum(axis=1)
+    h = 1e-4
+    for j in (0, 3, 7):
+        vals = []
+        for sgn in (1, -1):
+            gg = g.copy()
+            gg[j] += sgn * h
+            r = engine.evolve(n, s0, 0.0, dt, len(g) * sps, order=bk.BK_STRANG, precision=bk.BK_C128,
+                              e0=e0, g_steps=np.repeat(gg, sps), measure_every=0)
+            vals.append(r["p_less"][-1])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert gseg[j] == pytest.approx(fd, abs=1e-4), (j, gseg[j], fd)
+
+
+def test_backward_leaves_state_and_detects_stale_tape(bk, engine):
+    n = 13
+    _, s0, e0 = bk.make_trap(n, 1)
+    gs = np.full(30, 0.4)
+    engine.adjoint_forward(n, s0, gs, 0.05, precision=bk.BK_C128, e0=e0, checkpoint_every=8)
+    before = engine.download()
+    g1, _ = engine.adjoint_backward(1.0, 0.0)
+    after = engine.download()
+    assert np.array_equal(before, after)
+    g2, _ = engine.adjoint_backward(1.0, 0.0)  # the tape can be reused
+    assert np.array_equal(g1, g2)
+    engine.evolve(n, s0, 0.4, 0.05, 3, precision=bk.BK_C128)
+    with pytest.raises(bk.BKError) as ei:
+        engine.adjoint_backward(1.0, 0.0)
+    assert ei.value.code == -4
+    with pytest.raises(bk.BKError) as ei:
+        engine.adjoint_forward(n, s0, gs, 0.05, precision=bk.BK_C128, checkpoint_every=0)
+    assert ei.value.code == -1
+
+
+def test_autograd_function_and_adam(bk, engine):
+    """torch.autograd.Function: the segment gradient equals the per-step
+    gradient summed per segment, the (w_p, w_d) weights come from the incoming
+    gradient, and 50 Adam steps on -P_<(T) raise P_<(T)."""
+    torch = pytest.importorskip("torch")
+    ag = importlib.import_module("bernasconi-krylov_amd.autograd")
+    n, sps, dt = 14, 10, 0.02
+    _, s0, e0 = bk.make_trap(n, 4)
+    g0 = _fields(8, 77)
+    g = torch.tensor(g0, requires_grad=True)
+    obs = ag.trotter_observables(g, engine, n, s0, dt, sps, precision=bk.BK_C64, e0=e0,
+                                 checkpoint_every=16)
+    (2.0 * obs[0] - 0.5 * obs[1]).backward()
+    engine.adjoint_forward(n, s0, np.repeat(g0, sps), dt, precision=bk.BK_C64, e0=e0, checkpoint_every=16)
+    ref, _ = engine.adjoint_backward(2.0, -0.5)
+    assert np.allclose(g.grad.numpy(), ref.reshape(8, sps).sum(axis=1), rtol=0, atol=1e-12)
+    res = ag.optimize_schedule(engine, n, s0, g0, dt, sps, iters=50, lr=0.02, e0=e0,
+                               checkpoint_every=16)
+    assert res["loss"][-1] < res["loss"][0]
+    assert np.all(np.isfinite(res["g"]))

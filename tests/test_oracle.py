@@ -377,3 +377,37 @@ def test_schedule_counts(orc):
     assert int(((f & 2) > 0).sum()) == 5
     f = orc.schedule(3.0, 0.05, num_sample_points=10, shots_per_point=20, track_p_less=False)
     assert int((f & 1).sum()) == 0
+
+
+# ------------------------------------------------------------------ adjoint gradient
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("w", [(1.0, 0.0), (0.0, 1.0), (0.6, -0.25)])
+def test_adjoint_matches_central_differences(orc, order, w):
+    """bko_adjoint_grad against central differences of the forward evolution
+    (h = 1e-5; the difference error is O(h^2) ~ 1e-10)."""
+    n, s0, dt = 8, 0b10110010, 0.05
+    rng = np.random.default_rng(11 + order)
+    gs = 0.35 + 0.05 * rng.uniform(-1.0, 1.0, 10)
+    grad, (pl, dh) = orc.adjoint_grad(n, s0, gs, dt, order=order, wp=w[0], wd=w[1])
+    fwd = orc.trotter(n, s0, 0.0, dt, len(gs), order=order, g_steps=gs)
+    assert pl == pytest.approx(fwd["p_less"][-1], abs=1e-14)
+    assert dh == pytest.approx(fwd["dh"][-1], abs=1e-13)
+    h = 1e-5
+    for k in range(len(gs)):
+        gp, gm = gs.copy(), gs.copy()
+        gp[k] += h
+        gm[k] -= h
+        a = orc.trotter(n, s0, 0.0, dt, len(gs), order=order, g_steps=gp)
+        b = orc.trotter(n, s0, 0.0, dt, len(gs), order=order, g_steps=gm)
+        fd = (w[0] * (a["p_less"][-1] - b["p_less"][-1]) + w[1] * (a["dh"][-1] - b["dh"][-1])) / (2 * h)
+        assert grad[k] == pytest.approx(fd, abs=1e-9), k
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_adjoint_single_step_closed_form(orc, order):
+    """One step from |s0>: <d_H> = N sin^2(g dt) (the phases act on a basis
+    state or on |psi|^2 only), so d<d_H>/dg = N dt sin(2 g dt)."""
+    for n, g, dt in ((5, 0.3, 0.1), (9, -0.7, 0.05), (12, 1.3, 0.4)):
+        grad, (_, dh) = orc.adjoint_grad(n, 0b101 & ((1 << n) - 1), [g], dt, order=order, wp=0.0, wd=1.0)
+        assert dh == pytest.approx(n * math.sin(g * dt) ** 2, abs=1e-13)
+        assert grad[0] == pytest.approx(n * dt * math.sin(2 * g * dt), abs=1e-12)
