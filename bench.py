@@ -58,6 +58,54 @@ def _traffic(kernel: str, n: int, precision: str, energy: str):
     return None
 
 
+def _adjoint_bench(bk, eng, seed: int) -> dict:
+    """Differentiation workload: N=26 c64 Strang, 64 segments x 20 steps of
+    dt=0.02 with g_j = 0.35 +- 0.05 drawn from mt19937_64(seed), loss -P_<(T),
+    checkpoints every 64 steps.  Times the plain forward (bk_evolve, observables
+    at T only), the forward that records the tape, and the reverse pass."""
+    import numpy as np
+    import torch
+    n, nseg, sps, dt, K = 26, 64, 20, 0.02, 64
+    _, s0, e0 = bk.make_trap(n, seed)
+    g_steps = np.repeat(bk.field_schedule(seed, nseg), sps)
+    steps = nseg * sps
+    # warm: allocations, energy cache, tape
+    eng.adjoint_forward(n, s0, g_steps, dt, precision=bk.BK_C64, e0=e0, checkpoint_every=K)
+    eng.adjoint_backward(-1.0, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd = eng.evolve(n, s0, 0.0, dt, steps, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                     g_steps=g_steps, measure_every=0)
+    t1 = time.perf_counter()
+    obs = eng.adjoint_forward(n, s0, g_steps, dt, precision=bk.BK_C64, e0=e0, checkpoint_every=K)
+    t2 = time.perf_counter()
+    grad, drift = eng.adjoint_backward(-1.0, 0.0)  # loss = -P_<(T)
+    t3 = time.perf_counter()
+    eng.timing_enable(True)  # per-sweep device times from a second pass over the same tape
+    eng.adjoint_backward(-1.0, 0.0)
+    rows = eng.timing_read()
+    eng.timing_enable(False)
+    t_fwd, t_tape, t_bwd = t1 - t0, t2 - t1, t3 - t2
+    sweeps = {r["name"]: {"avg_ms": round(r["total_ms"] / r["launches"], 4), "launches": r["launches"],
+                          "gbs": round(r["bytes_per_launch"] / (r["total_ms"] / r["launches"]) * 1e-6, 1)}
+              for r in rows}
+    gseg = grad.reshape(nseg, sps).sum(axis=1)
+    return {
+        "workload": f"N={n} c64 Strang dt={dt}, {nseg} segments x {sps} steps, g=0.35+-0.05 "
+                    f"(mt19937_64 seed {seed}), loss -P_<(T), checkpoints every {K}",
+        "forward_s": t_fwd,
+        "forward_with_tape_s": t_tape,
+        "backward_s": t_bwd,
+        "backward_over_forward": t_bwd / t_fwd,
+        "gradient_over_forward": (t_tape + t_bwd) / t_fwd,
+        "p_less_T": obs[0],
+        "p_less_T_plain_forward": float(fwd["p_less"][-1]),
+        "max_abs_dL_dg_segment": float(np.abs(gseg).max()),
+        "reversal_drift": drift,
+        "reverse_sweeps": sweeps,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +120,8 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=30)
     ap.add_argument("--energy", choices=["cache16", "onthefly"], default="cache16",
                     help="how the first sweep obtains E(s)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the N=26 adjoint/forward timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 8 GiB device copy (HBM ceiling / PMC calibration)")
     args = ap.parse_args()
@@ -127,6 +177,11 @@ def main() -> int:
 
     copy_gbs = eng.copy_probe(1 << 33, 3)  # practical read+write ceiling on this device
 
+    adj = None
+    if not args.no_adjoint:
+        print("[bench] adjoint N=26", file=sys.stderr, flush=True)
+        adj = _adjoint_bench(bk, eng, args.seed)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -167,6 +222,7 @@ def main() -> int:
             "copy_ceiling_gbs": copy_gbs,
             "frac_of_copy_ceiling": achieved / copy_gbs,
         },
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

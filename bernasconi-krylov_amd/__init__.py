@@ -219,6 +219,47 @@ def make_trap(n: int, seed: int):
     return a.value, b.value, e.value
 
 
+class MT19937_64:
+    """std::mt19937_64 (its output sequence is fixed by the C++ standard), for
+    seed-drawn synthetic inputs on the host."""
+
+    _M = (1 << 64) - 1
+
+    def __init__(self, seed: int = 5489):
+        self.mt = [0] * 312
+        self.mt[0] = seed & self._M
+        for i in range(1, 312):
+            self.mt[i] = (6364136223846793005 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 62)) + i) & self._M
+        self.idx = 312
+
+    def __call__(self) -> int:
+        if self.idx >= 312:
+            mt = self.mt
+            for i in range(312):
+                y = (mt[i] & 0xFFFFFFFF80000000) | (mt[(i + 1) % 312] & 0x7FFFFFFF)
+                v = mt[(i + 156) % 312] ^ (y >> 1)
+                if y & 1:
+                    v ^= 0xB5026F5AA96619E9
+                mt[i] = v
+            self.idx = 0
+        x = self.mt[self.idx]
+        self.idx += 1
+        x ^= (x >> 29) & 0x5555555555555555
+        x ^= (x << 17) & 0x71D67FFFEDA60000
+        x ^= (x << 37) & 0xFFF7EEE000000000
+        x ^= x >> 43
+        return x & self._M
+
+
+def field_schedule(seed: int, n_segments: int, center: float = 0.35, spread: float = 0.05) -> np.ndarray:
+    """Piecewise-constant field g_j = center + spread * (2u - 1), u = (x >> 11) * 2^-53
+    with x drawn from mt19937_64(seed): a portable uniform (unlike
+    std::uniform_real_distribution, whose algorithm the standard leaves open)."""
+    rng = MT19937_64(seed)
+    u = np.array([(rng() >> 11) * 2.0 ** -53 for _ in range(n_segments)])
+    return center + spread * (2.0 * u - 1.0)
+
+
 def sweep_plan(n: int, precision: int = BK_C64, row_bits: int = 5, page_bits: int = 0):
     """HIGH sweeps of one Trotter step: list of (h1, a1, h2, a2, R)."""
     buf = (C.c_int * 40)()

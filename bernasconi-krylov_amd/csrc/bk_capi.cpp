@@ -164,6 +164,15 @@ struct bk_ctx {
     // bumped by every call that changes the held state; an adjoint tape is
     // valid only for the generation it was recorded at
     uint64_t state_gen = 0;
+    // bumped when the held state is set anew (init / upload): the rounding
+    // carries of the step-phase tables start over with each history
+    uint64_t history_id = 0;
+    struct FbTab {
+        DevBuf master, carry, applied;
+        int count = 0;
+        uint64_t hist = ~0ull;
+    };
+    std::map<std::pair<int, std::pair<int, double>>, FbTab> fb_tabs;  // (n, (prec, dtp))
     // adjoint tape: the forward run's parameters and its checkpoint states
     struct Tape {
         bool valid = false;
@@ -236,6 +245,51 @@ int get_phase_tab(bk_ctx* c, int n, int precision, double dtp, const void** out)
     return BK_OK;
 }
 
+// Step-phase table with error feedback: exp(-i dtp E/n) in FP64 (master), the
+// per-entry rounding carry, and the rounded table the sweeps read.  The carry
+// restarts (all ones) the first time the table is used in a new history.
+int get_fb_tab(bk_ctx* c, int n, int precision, double dtp, bk_ctx::FbTab** out)
+{
+    auto key = std::make_pair(n, std::make_pair(precision, dtp));
+    auto it = c->fb_tabs.find(key);
+    if (it == c->fb_tabs.end()) {
+        if (c->fb_tabs.size() >= 16) {  // bound the cache (a cleared table restarts its carry)
+            for (auto& kv : c->fb_tabs) {
+                kv.second.master.release();
+                kv.second.carry.release();
+                kv.second.applied.release();
+            }
+            c->fb_tabs.clear();
+        }
+        const int64_t emax = static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6;
+        int64_t codd = 0;
+        for (int k = 1; k < n; ++k) codd += (n - k) & 1;
+        const int64_t c4 = codd % 4;
+        const int cnt = static_cast<int>(emax >> 2) + 1;
+        const double w = -(dtp / static_cast<double>(n));
+        std::vector<double> h(2 * static_cast<size_t>(cnt));
+        for (int m = 0; m < cnt; ++m) {
+            const double th = w * static_cast<double>(4 * static_cast<int64_t>(m) + c4);
+            h[2 * m] = std::cos(th);
+            h[2 * m + 1] = std::sin(th);
+        }
+        bk_ctx::FbTab t;
+        HIPCALL(t.master.ensure(h.size() * sizeof(double)));
+        HIPCALL(t.carry.ensure(h.size() * sizeof(double)));
+        HIPCALL(t.applied.ensure(static_cast<size_t>(cnt) * amp_bytes(precision)));
+        HIPCALL(hipMemcpy(t.master.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        t.count = cnt;
+        it = c->fb_tabs.emplace(key, t).first;
+    }
+    bk_ctx::FbTab& t = it->second;
+    if (t.hist != c->history_id) {
+        HIPCALL(bk::launch_carry_reset(t.carry.as<double>(), t.count, c->stream));
+        t.hist = c->history_id;
+    }
+    *out = &t;
+    return BK_OK;
+}
+
 int ensure_state(bk_ctx* c, int n, int precision)
 {
     if (n < 1 || n > BK_MAX_N) return fail(BK_E_INVALID, "n=%d outside [1, %d]", n, BK_MAX_N);
@@ -250,6 +304,7 @@ int ensure_state(bk_ctx* c, int n, int precision)
     c->has_state = true;
     c->theta_pre = 0.0;
     ++c->state_gen;
+    ++c->history_id;
     if (!c->tuning_pinned) c->tuning = bk::default_tuning(n, precision);
     c->tuning.timer = c->timing ? &c->timer : nullptr;
     HIPCALL(c->partials.ensure(bk::partials_bytes(c->tuning.grid)));
@@ -331,6 +386,11 @@ void bk_ctx_destroy(bk_ctx* c)
     c->adj_grad.release();
     c->adj_drift.release();
     for (auto& kv : c->phase_tabs) kv.second.release();
+    for (auto& kv : c->fb_tabs) {
+        kv.second.master.release();
+        kv.second.carry.release();
+        kv.second.applied.release();
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -494,15 +554,18 @@ static int ensure_ecache(bk_ctx* c)
 // unchanged.  slot: device pointer for the stepped state's observables, or null.
 static int enqueue_step(bk_ctx* c, double dt, double g, uint64_t s0, uint32_t e0, double* slot)
 {
-    const void* tab = nullptr;
-    int rc = get_phase_tab(c, c->n, c->precision, dt, &tab);
+    bk_ctx::FbTab* fb = nullptr;
+    int rc = get_fb_tab(c, c->n, c->precision, dt, &fb);
     if (rc) return rc;
     bk::StepSpec sp{};
     sp.n = c->n;
     sp.precision = c->precision;
     sp.gdt = g * dt;
     sp.dtp = dt;
-    sp.phase_tab = tab;
+    sp.phase_tab = fb->applied.p;
+    sp.fb_master = fb->master.as<double>();
+    sp.fb_carry = fb->carry.as<double>();
+    sp.fb_count = fb->count;
     sp.series_slot = slot;
     sp.s0 = static_cast<uint32_t>(s0);
     sp.e0 = e0;

@@ -587,6 +587,36 @@ __global__ __launch_bounds__(kThreads) void k_finish_u64(const unsigned long lon
     if (threadIdx.x == 0) *out = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
 }
 
+// Error-feedback phase table: applied[m] = round(sd * master[m] * carry[m]),
+// carry[m] = exact target / applied (FP64 complex).
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_phase_feedback(const double2* __restrict__ master,
+                                                            double2* __restrict__ carry, V* __restrict__ applied,
+                                                            int count, double sd)
+{
+    using R = typename RealOf<V>::T;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < count; m += gridDim.x * blockDim.x) {
+        const double2 w = master[m], c = carry[m];
+        const double tr = sd * (w.x * c.x - w.y * c.y), ti = sd * (w.x * c.y + w.y * c.x);
+        V a;
+        a.x = static_cast<R>(tr);
+        a.y = static_cast<R>(ti);
+        const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
+        const double inv = 1.0 / (ar * ar + ai * ai);
+        double2 nc;
+        nc.x = (tr * ar + ti * ai) * inv;  // target / a
+        nc.y = (ti * ar - tr * ai) * inv;
+        carry[m] = nc;
+        applied[m] = a;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_carry_reset(double2* __restrict__ carry, int count)
+{
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < count; m += gridDim.x * blockDim.x)
+        carry[m] = make_double2(1.0, 0.0);
+}
+
 // sum 'count' doubles in a fixed order into *out
 __global__ __launch_bounds__(kThreads) void k_finish_sum(const double* __restrict__ partials, int count,
                                                         double* __restrict__ out)
@@ -836,11 +866,27 @@ hipError_t step_impl(V* psi, V* alt, int* in_alt, const StepSpec& sp, const Swee
     // [0, TB)) which ends with scale * exp(-i dtp E/n) and the reductions.
     *in_alt = 0;
     using C = Cfg<V>;
+    using Rt = typename RealOf<V>::T;
     const int n = sp.n;
-    const MixCoef mc = mix_coef(n, sp.gdt);
+    MixCoef mc = mix_coef(n, sp.gdt);
     const uint64_t dim = 1ull << n;
     const bool measure = sp.series_slot != nullptr;
     hipError_t e = hipSuccess;
+    if (sp.fb_carry) {
+        // norm-exact scale for the coefficients as the kernels see them, folded
+        // into the error-feedback table (the sweeps then apply scale 1)
+        const double b = static_cast<double>(static_cast<Rt>(mc.bs));
+        const double sd = mc.tan_form ? std::pow(1.0 + b * b, -0.5 * n)
+                                      : std::pow(static_cast<double>(static_cast<Rt>(mc.bc)) *
+                                                     static_cast<double>(static_cast<Rt>(mc.bc)) + b * b,
+                                                 -0.5 * n);
+        const int fg = std::max(1, std::min((sp.fb_count + kThreads - 1) / kThreads, 64));
+        hipLaunchKernelGGL(k_phase_feedback<V>, dim3(fg), dim3(kThreads), 0, st,
+                           reinterpret_cast<const double2*>(sp.fb_master), reinterpret_cast<double2*>(sp.fb_carry),
+                           static_cast<V*>(const_cast<void*>(sp.phase_tab)), sp.fb_count, sd);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        mc.scale = 1.0;
+    }
     if (n < C::TB) {
         // small states: one kernel per qubit, then elementwise scale*phase (+measure)
         const uint64_t np = dim / 2;
@@ -1210,6 +1256,13 @@ hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, doubl
                                               static_cast<float2*>(psi), s0, re, im);
     else hipLaunchKernelGGL(k_set_basis<double2>, dim3(1), dim3(1), 0, stream, static_cast<double2*>(psi), s0,
                             re, im);
+    return hipGetLastError();
+}
+
+hipError_t launch_carry_reset(double* carry, int count, hipStream_t stream)
+{
+    const int g = std::max(1, std::min((count + kThreads - 1) / kThreads, 64));
+    hipLaunchKernelGGL(k_carry_reset, dim3(g), dim3(kThreads), 0, stream, reinterpret_cast<double2*>(carry), count);
     return hipGetLastError();
 }
 

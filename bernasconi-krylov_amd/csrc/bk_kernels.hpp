@@ -31,6 +31,16 @@ struct StepSpec {
     // optional uint16 energy cache (E(s) per amplitude); null -> energies are
     // computed on the fly with popcounts in the sweep
     const uint16_t* ecache = nullptr;
+    // Optional error-feedback phase table (fb_carry non-null): before the step's
+    // phase multiply, entry m of phase_tab (writable, fb_count entries) is set
+    // to round(sd * fb_master[m] * fb_carry[m]) and fb_carry[m] to the exact
+    // quotient target / rounded, in FP64; sd is the step's norm-exact mixer
+    // scale.  The rounding residual of every entry is carried into the next
+    // step, so the product of the applied factors follows the exact product to
+    // one rounding instead of drifting by one rounding per step.
+    const double* fb_master = nullptr;  // double2 exp(-i dtp E_m/n) per entry
+    double* fb_carry = nullptr;         // double2 per entry
+    int fb_count = 0;
 };
 
 // Optional per-launch timing hook (HIP events recorded on the launch stream).
@@ -91,6 +101,9 @@ hipError_t launch_phase(void* psi, int n, int precision, const void* phase_tab, 
 // psi = (re + i im) |s0>
 hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, double re, double im,
                              hipStream_t stream);
+
+// fb_carry[m] = 1 for m < count (start of a new error-feedback history)
+hipError_t launch_carry_reset(double* carry, int count, hipStream_t stream);
 
 // E(s) for all 2^n states (uint16, exact for n <= 32) in the LOW sweep's tile
 // layout for 'precision' (the layout differs between c64 and c128)

@@ -155,3 +155,21 @@ def test_cxx_dropin_headers_compile_and_link(bk, tmp_path):
     assert lines[0] == "cfg 16 0.5 0.05 60 20 100 42"
     assert lines[1] == "diag 1 100 1e-12"
     assert "big ok" in lines
+
+
+def test_host_mt19937_64_and_field_schedule(bk, orc):
+    """The package's mt19937_64 (used for seed-drawn fields) against the C++
+    standard's fixed value (10000th output of the default seed) and the
+    oracle's generator; the field schedule uses the portable (x>>11)*2^-53."""
+    g = bk.MT19937_64()
+    for _ in range(9999):
+        g()
+    assert g() == 9981545732273789042
+    a, b = bk.MT19937_64(42), orc.MT64(42)
+    for _ in range(700):  # crosses a state refill
+        assert a() == b.next()
+    f = bk.field_schedule(7, 64)
+    c = bk.MT19937_64(7)
+    ref = [0.35 + 0.05 * (2.0 * ((c() >> 11) * 2.0 ** -53) - 1.0) for _ in range(64)]
+    assert f.tolist() == ref
+    assert f.min() >= 0.30 and f.max() <= 0.40

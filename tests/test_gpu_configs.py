@@ -1,0 +1,89 @@
+"""The named workloads, checked end to end on the GPU:
+
+* N=12 plain method: Lie steps of dt=0.05 against the oracle's Trotter loop and
+  against the exact evolution exp(-iHt) (dense ED fixture), with the Trotter
+  error shrinking as dt^2 (P_< and <d_H> are diagonal, where Lie and Strang agree);
+* N=24, d_min >= 6 trap: 1000 Strang steps in complex64 against complex128
+  (P_<, <d_H> every 10 steps; state overlap every 100);
+* N=26: P_<(T) of Strang dt=0.02 against the FP64 matrix-free Krylov
+  propagator (m = 30).  At N=26 the spectrum of H = E/N - g sum X spans ~220,
+  and restarting every 0.5 leaves the m=30 Krylov step unconverged (measured:
+  P_<(40) = 0.011593 against 0.012058 from restarts every 0.25, which agrees
+  with the dt -> 0 Richardson limit of the Trotter runs to 1e-10); the test
+  restarts every 0.25 and stops at T = 10 to stay within a minute.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_n12_lie_against_exact_evolution(bk, orc, engine, golden, prec):
+    d = golden("ed_n12.json")
+    n, g = d["n"], d["g"]
+    assert bk.make_trap(n, d["seed"]) == (d["start"], d["s0"], d["e0"])
+    s0, e0 = d["s0"], d["e0"]
+    precision = bk.BK_C128 if prec == "c128" else bk.BK_C64
+    ed_p, ed_d = np.array(d["p_less"][:201]), np.array(d["dh"][:201])
+    r = engine.evolve(n, s0, g, 0.05, 200, order=bk.BK_LIE, precision=precision, e0=e0)
+    tr = orc.trotter(n, s0, g, 0.05, 200, order=1, e0=e0)
+    tol = (1e-12, 1e-11) if prec == "c128" else (5e-7, 5e-6)
+    assert np.abs(r["p_less"] - tr["p_less"]).max() <= tol[0]
+    assert np.abs(r["dh"] - tr["dh"]).max() <= tol[1]
+    e1p, e1d = np.abs(r["p_less"] - ed_p).max(), np.abs(r["dh"] - ed_d).max()
+    assert e1p <= 5e-4 and e1d <= 5e-3, (e1p, e1d)
+    r2 = engine.evolve(n, s0, g, 0.025, 400, order=bk.BK_LIE, precision=precision, e0=e0)
+    e2p, e2d = np.abs(r2["p_less"][::2] - ed_p).max(), np.abs(r2["dh"][::2] - ed_d).max()
+    assert 3.0 < e1p / e2p < 5.0 and 3.0 < e1d / e2d < 5.0, (e1p / e2p, e1d / e2d)
+
+
+def test_n24_c64_against_c128(bk):
+    seed, start, s0, e0, dm = bk.make_trap_min_dmin(24, 1, 6)
+    assert dm >= 6
+    e64, e128 = bk.Engine(0), bk.Engine(0)
+    try:
+        dp, dd, ov = [], [], []
+        for chunk in range(10):
+            kw = dict(order=bk.BK_STRANG, e0=e0, init=chunk == 0, measure_every=10)
+            a = e64.evolve(24, s0, 0.35, 0.02, 100, precision=bk.BK_C64, **kw)
+            b = e128.evolve(24, s0, 0.35, 0.02, 100, precision=bk.BK_C128, **kw)
+            dp.append(np.abs(a["p_less"] - b["p_less"]).max())
+            dd.append(np.abs(a["dh"] - b["dh"]).max())
+            psi64 = e64.download().astype(np.complex128)
+            psi128 = e128.download()
+            ov.append(1.0 - abs(np.vdot(psi128, psi64)))
+        print(f"\n[c64 vs c128, N=24, seed {seed}] max|dP_<| per 100 steps {np.array(dp)}")
+        print(f"  max|d<d_H>| {np.array(dd)}\n  1-|<psi64|psi128>| {np.array(ov)}")
+        assert max(dp) <= 1e-5
+        assert max(dd) <= 1e-4
+        assert max(ov) <= 1e-6
+    finally:
+        e64.close()
+        e128.close()
+
+
+@pytest.mark.slow
+def test_n26_trotter_against_krylov(bk, engine):
+    n, g, T = 26, 0.35, 10.0
+    _, s0, e0 = bk.make_trap(n, 1)
+    kr = bk.Engine(0)
+    try:
+        kr.state_init(n, bk.BK_C128, s0)
+        kr.krylov_propagate(g, T, tau_restart=0.25, m=30)
+        pk, dk, nk = kr.observables(s0, e0)
+    finally:
+        kr.close()
+    assert nk == pytest.approx(1.0, abs=1e-10)
+    steps = int(round(T / 0.02))
+    r128 = engine.evolve(n, s0, g, 0.02, steps, order=bk.BK_STRANG, precision=bk.BK_C128, e0=e0,
+                         measure_every=0)
+    r64 = engine.evolve(n, s0, g, 0.02, steps, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                        measure_every=0)
+    p128, p64 = r128["p_less"][-1], r64["p_less"][-1]
+    print(f"\n[N=26 T={T}] Krylov P_< {pk:.10f}  Trotter c128 {p128:.10f}  c64 {p64:.10f}")
+    assert abs(p128 - pk) <= 2e-5
+    assert abs(p64 - pk) <= 2e-5
+    assert abs(p64 - p128) <= 1e-5
