@@ -60,6 +60,24 @@ class AdjointParams(C.Structure):
     _fields_ = [("ev", EvolveParams), ("checkpoint_every", C.c_int)]
 
 
+class Instance(C.Structure):
+    _fields_ = [("n", C.c_int), ("start_bits", C.c_uint64), ("e0", C.c_int64), ("g", C.c_double)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("order", C.c_int), ("dt", C.c_double), ("steps", C.c_int),
+                ("use_graph", C.c_int), ("autotune", C.c_int)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("p_less", C.c_double), ("dh", C.c_double), ("norm", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("grid", C.c_int), ("row_bits", C.c_int),
+                ("block_bytes", C.c_int64), ("launches", C.c_int64), ("tuned", C.c_int)]
+
+
 class Series(C.Structure):
     _fields_ = [("p_less", C.POINTER(C.c_double)), ("dh", C.POINTER(C.c_double)),
                 ("norm", C.POINTER(C.c_double)), ("capacity", C.c_int64), ("count", C.c_int64),
@@ -128,6 +146,8 @@ _SIGS = {
     "bk_adjoint_grad": (C.c_int, [C.c_void_p, C.POINTER(AdjointParams), C.c_double, C.c_double,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
+    "bk_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Instance), C.c_int, C.POINTER(BatchParams),
+                               C.POINTER(BatchResult), C.POINTER(BatchStats)]),
     "bk_observables": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.POINTER(C.c_double)]),
     "bk_count_below": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_uint64)]),
     "bk_sample": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
@@ -260,6 +280,23 @@ def field_schedule(seed: int, n_segments: int, center: float = 0.35, spread: flo
     return center + spread * (2.0 * u - 1.0)
 
 
+def batch_workload(count: int = 512, seed: int = 1, trap_base: int = 1000, sizes=(16, 18, 20, 22),
+                   g_lo: float = 0.10, g_hi: float = 0.60, n_g: int = 16):
+    """Seed-drawn batch of traps: instance i has n_i = sizes[x >> 62] for successive
+    draws x of mt19937_64(seed) (a portable uniform choice among four sizes),
+    g_i = g_lo + (g_hi - g_lo) * (i mod n_g) / (n_g - 1), and its trap from seed
+    trap_base + i.  Returns a list of (n, s0, E0, g)."""
+    if len(sizes) != 4:
+        raise ValueError("four sizes")
+    rng = MT19937_64(seed)
+    out = []
+    for i in range(count):
+        n = sizes[rng() >> 62]
+        _, s0, e0 = make_trap(n, trap_base + i)
+        out.append((n, s0, e0, g_lo + (g_hi - g_lo) * (i % n_g) / (n_g - 1)))
+    return out
+
+
 def sweep_plan(n: int, precision: int = BK_C64, row_bits: int = 5, page_bits: int = 0):
     """HIGH sweeps of one Trotter step: list of (h1, a1, h2, a2, R)."""
     buf = (C.c_int * 40)()
@@ -351,6 +388,24 @@ class Engine:
         k = s.count
         return {"p_less": pl[:k], "dh": dh[:k], "norm": nm[:k], "e0": s.e0,
                 "p_blind": s.p_blind, "_keep": gs}
+
+    # ---- batched evolution ----
+    def batch_run(self, instances, dt: float, steps: int, order: int = BK_STRANG,
+                  precision: int = BK_C64, use_graph: bool = True, autotune: bool = False):
+        """Evolve many traps at once.  instances: iterable of (n, start_bits, e0, g)
+        (e0 < 0 -> E(start_bits)).  Returns (array count x 3 of P_<, <d_H>, norm after
+        the last step, stats dict)."""
+        inst = list(instances)
+        arr = (Instance * len(inst))(*[Instance(int(n), int(s), int(e), float(g)) for n, s, e, g in inst])
+        res = (BatchResult * len(inst))()
+        st = BatchStats()
+        p = BatchParams(precision=precision, order=order, dt=dt, steps=steps,
+                        use_graph=1 if use_graph else 0, autotune=1 if autotune else 0)
+        _check(lib().bk_batch_run(self._h, arr, len(inst), C.byref(p), res, C.byref(st)))
+        out = np.array([[r.p_less, r.dh, r.norm] for r in res])
+        stats = {"device_ms": st.device_ms, "grid": st.grid, "row_bits": st.row_bits,
+                 "block_bytes": st.block_bytes, "launches": st.launches, "tuned": bool(st.tuned)}
+        return out, stats
 
     # ---- adjoint gradient ----
     def adjoint_forward(self, n: int, start_bits: int, g_steps: Sequence[float], dt: float,

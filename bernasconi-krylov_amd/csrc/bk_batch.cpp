@@ -1,0 +1,402 @@
+// bk_batch.cpp -- bk_batch_run: many traps of small n in shared launches.
+//
+// The reference evolves one trap per ISampler::sample call (sampler.cuh:412-707);
+// at n = 16..22 a single state is 0.5-32 MB and one Trotter step of it is a few
+// microseconds of HBM time, so one-trap-at-a-time runs are bound by launch and
+// latency (measured 16-33 us per step).  Here all instances of one n share every
+// launch: a sweep's tiles run over the states stored one after another, each
+// state with its own error-feedback phase table, field and references.
+//
+// Order of work.  Without cache blocking every step visits all states (one
+// feedback update, the high-qubit sweeps and the LOW sweep per (n, butterfly
+// form) group).  With a cache block of B bytes, the states are cut into blocks
+// of at most B bytes and each block runs all its steps before the next starts,
+// so a block that fits the 256 MB last-level cache is re-read from there.
+// The whole sequence can be captured once as a hipGraph and launched once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/bk_capi.h"
+#include "bk_ctx.hpp"
+#include "bk_host.hpp"
+#include "bk_kernels.hpp"
+#include "bk_sweeps.hpp"
+
+using namespace bkc;
+
+namespace {
+
+struct Group {
+    int n = 0;
+    bool tan = true;
+    std::vector<int> idx;  // instance indices (input order within the group)
+    size_t state_off = 0;  // amplitude offset in the state arena
+    size_t tab_off = 0;    // entry offset in the table arenas
+    size_t arr_off = 0;    // offset in the per-state arrays
+    size_t part_off = 0;   // partial-triple offset
+    int tab_count = 0;     // entries per table
+    size_t master_off = 0; // double offset in the master arena
+};
+
+struct Tune {
+    int grid, row_bits;
+    int64_t block_bytes;
+};
+
+// device pointer of a named arena buffer of at least 'bytes'
+int arena(bk_ctx* c, const char* name, size_t bytes, void** out)
+{
+    DevBuf& b = c->batch_bufs[name];
+    if (b.cap < bytes) {
+        b.release();
+        HIPCALL(b.ensure(std::max<size_t>(bytes, 256)));
+    }
+    *out = b.p;
+    return BK_OK;
+}
+
+struct Plan {
+    std::vector<Group> groups;
+    int precision = 0;
+    size_t esz = 0;
+    char* states = nullptr;
+    char* applied = nullptr;
+    double* carry = nullptr;
+    double* master = nullptr;
+    double *bc = nullptr, *bs = nullptr, *sd = nullptr, *re = nullptr, *im = nullptr;
+    uint32_t *s0 = nullptr, *e0 = nullptr;
+    double* partials = nullptr;
+    double* results = nullptr;
+    std::map<int, const uint16_t*> ecache;
+    size_t total_states_bytes = 0;
+};
+
+// enqueue one run of 'steps' steps (starting states prepared by init()) with
+// tuning t; counts the launches
+int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* launches)
+{
+    bk::SweepTuning tun = bk::default_tuning(20, P.precision);
+    tun.grid = t.grid;
+    tun.row_bits = t.row_bits;
+    tun.timer = nullptr;
+    int64_t nl = 0;
+    auto group_block = [&](const Group& g, int first, int count) {
+        bk::BatchGroup b{};
+        b.n = g.n;
+        b.precision = P.precision;
+        b.nstates = count;
+        b.psi = P.states + (g.state_off + (static_cast<size_t>(first) << g.n)) * P.esz;
+        b.tab = P.applied + (g.tab_off + static_cast<size_t>(first) * g.tab_count) * P.esz;
+        b.tab_stride = g.tab_count;
+        b.ecache = P.ecache.at(g.n);
+        b.bc = P.bc + g.arr_off + first;
+        b.bs = P.bs + g.arr_off + first;
+        b.s0 = P.s0 + g.arr_off + first;
+        b.e0 = P.e0 + g.arr_off + first;
+        b.tan_form = g.tan;
+        return b;
+    };
+    auto one_step = [&](const Group& g, int first, int count, bool last) -> int {
+        HIPCALL(bk::launch_batch_feedback(P.master + g.master_off,
+                                          P.carry + 2 * (g.tab_off + static_cast<size_t>(first) * g.tab_count),
+                                          P.applied + (g.tab_off + static_cast<size_t>(first) * g.tab_count) * P.esz,
+                                          g.tab_count, count, P.sd + g.arr_off + first, P.precision, c->stream));
+        const bk::BatchGroup b = group_block(g, first, count);
+        const size_t tps = static_cast<size_t>(1) << (g.n - bk::low_tile_bits(P.precision));
+        double* part = last ? P.partials + 3 * (g.part_off + static_cast<size_t>(first) * tps) : nullptr;
+        HIPCALL(bk::launch_batch_step(b, tun, part, c->stream));
+        nl += 2 + static_cast<int64_t>(bk::plan_high_sweeps(g.n, bk::low_tile_bits(P.precision),
+                                                            bk::low_tile_bits(P.precision) + 1,
+                                                            std::max(tun.row_bits, 3), tun.page_bits)
+                                           .size());
+        return BK_OK;
+    };
+    int rc = BK_OK;
+    if (t.block_bytes <= 0) {
+        for (int k = 0; k < steps; ++k)
+            for (const Group& g : P.groups)
+                if ((rc = one_step(g, 0, static_cast<int>(g.idx.size()), k + 1 == steps))) return rc;
+    } else {
+        for (const Group& g : P.groups) {
+            const int64_t sb = static_cast<int64_t>(P.esz) << g.n;
+            const int per = static_cast<int>(std::max<int64_t>(1, t.block_bytes / sb));
+            const int ns = static_cast<int>(g.idx.size());
+            for (int first = 0; first < ns; first += per) {
+                const int cnt = std::min(per, ns - first);
+                for (int k = 0; k < steps; ++k)
+                    if ((rc = one_step(g, first, cnt, k + 1 == steps))) return rc;
+            }
+        }
+    }
+    for (const Group& g : P.groups) {
+        const int tps = 1 << (g.n - bk::low_tile_bits(P.precision));
+        HIPCALL(bk::launch_batch_finish(P.partials + 3 * g.part_off, tps, static_cast<int>(g.idx.size()),
+                                        P.results + 3 * g.arr_off, c->stream));
+        ++nl;
+    }
+    if (launches) *launches = nl;
+    return BK_OK;
+}
+
+// starting states: zero, the opening-phase amplitude at s0, carries reset
+int init_states(bk_ctx* c, const Plan& P)
+{
+    HIPCALL(hipMemsetAsync(P.states, 0, P.total_states_bytes, c->stream));
+    for (const Group& g : P.groups)
+        HIPCALL(bk::launch_batch_init(P.states + g.state_off * P.esz, g.n, static_cast<int>(g.idx.size()), P.precision,
+                                      P.s0 + g.arr_off, P.re + g.arr_off, P.im + g.arr_off, c->stream));
+    size_t ncarry = 0;
+    for (const Group& g : P.groups) ncarry += g.idx.size() * static_cast<size_t>(g.tab_count);
+    HIPCALL(bk::launch_carry_reset(P.carry, static_cast<int>(ncarry), c->stream));
+    return BK_OK;
+}
+
+std::string shape_key(const bk_batch_params* p, const std::vector<Group>& groups)
+{
+    std::string k = "p" + std::to_string(p->precision);
+    for (const Group& g : groups) k += ":" + std::to_string(g.n) + (g.tan ? "t" : "c") + std::to_string(g.idx.size());
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_params* p, bk_batch_result* out,
+                 bk_batch_stats* stats)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!inst || !p || !out || count < 1) return fail(BK_E_INVALID, "bk_batch_run: bad argument");
+    if (p->precision != BK_C64 && p->precision != BK_C128) return fail(BK_E_INVALID, "precision=%d", p->precision);
+    if (p->order != BK_LIE && p->order != BK_STRANG) return fail(BK_E_INVALID, "order=%d", p->order);
+    if (p->steps < 1 || !is_fin(p->dt)) return fail(BK_E_INVALID, "steps=%d dt=%g", p->steps, p->dt);
+    const int prec = p->precision;
+    const int tb = bk::low_tile_bits(prec);
+    for (int i = 0; i < count; ++i)
+        if (inst[i].n < tb || inst[i].n > BK_MAX_N || !is_fin(inst[i].g))
+            return fail(BK_E_INVALID, "instance %d: n=%d outside [%d, %d] or bad g", i, inst[i].n, tb, BK_MAX_N);
+
+    // ---- groups by (n, butterfly form) ----
+    std::map<std::pair<int, int>, Group> gm;
+    std::vector<bk::MixInfo> mix(static_cast<size_t>(count));
+    for (int i = 0; i < count; ++i) {
+        mix[static_cast<size_t>(i)] = bk::mix_info(inst[i].n, inst[i].g * p->dt, prec);
+        const bool tan = mix[static_cast<size_t>(i)].tan_form;
+        Group& g = gm[{inst[i].n, tan ? 1 : 0}];
+        g.n = inst[i].n;
+        g.tan = tan;
+        g.idx.push_back(i);
+    }
+    Plan P;
+    P.precision = prec;
+    P.esz = prec == BK_C64 ? 8 : 16;
+    size_t st_off = 0, tab_off = 0, arr_off = 0, part_off = 0, master_off = 0;
+    std::map<int, size_t> master_at;
+    std::vector<double> master_h;
+    for (auto& kv : gm) {
+        Group g = kv.second;
+        g.state_off = st_off;
+        g.arr_off = arr_off;
+        g.tab_off = tab_off;
+        g.part_off = part_off;
+        auto mit = master_at.find(g.n);
+        if (mit == master_at.end()) {
+            const std::vector<double> m = phase_master(g.n, p->dt);
+            master_at[g.n] = master_off;
+            master_h.insert(master_h.end(), m.begin(), m.end());
+            master_off += m.size();
+            mit = master_at.find(g.n);
+        }
+        g.master_off = mit->second;
+        g.tab_count = static_cast<int>(phase_master(g.n, p->dt).size() / 2);
+        const size_t ns = g.idx.size();
+        st_off += ns << g.n;
+        arr_off += ns;
+        tab_off += ns * static_cast<size_t>(g.tab_count);
+        part_off += ns << (g.n - tb);
+        P.groups.push_back(g);
+    }
+    P.total_states_bytes = st_off * P.esz;
+
+    // ---- arenas ----
+    void* v = nullptr;
+    if ((rc = arena(c, "states", P.total_states_bytes, &v))) return rc;
+    P.states = static_cast<char*>(v);
+    if ((rc = arena(c, "applied", tab_off * P.esz, &v))) return rc;
+    P.applied = static_cast<char*>(v);
+    if ((rc = arena(c, "carry", tab_off * 16, &v))) return rc;
+    P.carry = static_cast<double*>(v);
+    if ((rc = arena(c, "master", master_h.size() * 8, &v))) return rc;
+    P.master = static_cast<double*>(v);
+    if ((rc = arena(c, "dparams", arr_off * 5 * 8, &v))) return rc;
+    P.bc = static_cast<double*>(v);
+    P.bs = P.bc + arr_off;
+    P.sd = P.bs + arr_off;
+    P.re = P.sd + arr_off;
+    P.im = P.re + arr_off;
+    if ((rc = arena(c, "uparams", arr_off * 2 * 4, &v))) return rc;
+    P.s0 = static_cast<uint32_t*>(v);
+    P.e0 = P.s0 + arr_off;
+    if ((rc = arena(c, "partials", part_off * 3 * 8, &v))) return rc;
+    P.partials = static_cast<double*>(v);
+    if ((rc = arena(c, "results", arr_off * 3 * 8, &v))) return rc;
+    P.results = static_cast<double*>(v);
+    for (const Group& g : P.groups) {
+        DevBuf& ec = c->batch_ecache[{g.n, prec}];
+        if (!ec.p) {
+            HIPCALL(ec.ensure(sizeof(uint16_t) << g.n));
+            HIPCALL(bk::launch_energy_fill(ec.as<uint16_t>(), g.n, prec, c->stream));
+        }
+        P.ecache[g.n] = ec.as<uint16_t>();
+    }
+
+    // ---- per-state parameters ----
+    const double phi0 = p->order == BK_STRANG ? 0.5 * p->dt : p->dt;
+    std::vector<double> hd(arr_off * 5);
+    std::vector<uint32_t> hu(arr_off * 2);
+    for (const Group& g : P.groups)
+        for (size_t j = 0; j < g.idx.size(); ++j) {
+            const bk_instance& in = inst[g.idx[j]];
+            const bk::MixInfo& m = mix[static_cast<size_t>(g.idx[j])];
+            const size_t a = g.arr_off + j;
+            const uint64_t s0 = in.start_bits & ((1ull << in.n) - 1ull);
+            const int64_t es0 = bk::host_energy(in.n, s0);
+            const int64_t e0 = in.e0 >= 0 ? in.e0 : es0;
+            const double th = -(phi0 / static_cast<double>(in.n)) * static_cast<double>(es0);
+            hd[a] = m.bc;
+            hd[arr_off + a] = m.bs;
+            hd[2 * arr_off + a] = m.sd;
+            hd[3 * arr_off + a] = std::cos(th);
+            hd[4 * arr_off + a] = std::sin(th);
+            hu[a] = static_cast<uint32_t>(s0);
+            hu[arr_off + a] = static_cast<uint32_t>(std::max<int64_t>(0, std::min<int64_t>(e0, 0xFFFFFFFFll)));
+        }
+    HIPCALL(hipMemcpyAsync(P.master, master_h.data(), master_h.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(hipMemcpyAsync(P.bc, hd.data(), hd.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(hipMemcpyAsync(P.s0, hu.data(), hu.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));  // host vectors go out of scope below
+
+    // ---- tuning ----
+    Tune t{4096, prec == BK_C64 ? 5 : 4, 0};
+    const std::string key = shape_key(p, P.groups);
+    bool tuned = false;
+    auto tit = c->batch_tuned.find(key);
+    if (tit != c->batch_tuned.end()) {
+        t = {tit->second.grid, tit->second.row_bits, tit->second.block_bytes};
+        tuned = true;
+    } else if (p->autotune) {
+        hipEvent_t e0, e1;
+        HIPCALL(hipEventCreate(&e0));
+        HIPCALL(hipEventCreate(&e1));
+        const int trial_steps = std::min(p->steps, 8);
+        auto time_of = [&](const Tune& tt, float* ms) -> int {
+            int r = init_states(c, P);
+            if (r) return r;
+            HIPCALL(hipEventRecord(e0, c->stream));
+            if ((r = enqueue_run(c, P, trial_steps, tt, nullptr))) return r;
+            HIPCALL(hipEventRecord(e1, c->stream));
+            HIPCALL(hipEventSynchronize(e1));
+            HIPCALL(hipEventElapsedTime(ms, e0, e1));
+            return BK_OK;
+        };
+        float best = 0.f;
+        if ((rc = time_of(t, &best))) return rc;
+        // coordinate descent: grid, then row bits, then cache block
+        for (int gsz : {1024, 2048, 8192, 16384}) {
+            Tune cand = t;
+            cand.grid = gsz;
+            float ms = 0.f;
+            if ((rc = time_of(cand, &ms))) return rc;
+            if (ms < best) best = ms, t = cand;
+        }
+        for (int rb : {3, 4, 5, 6}) {
+            if (rb == t.row_bits) continue;
+            Tune cand = t;
+            cand.row_bits = rb;
+            float ms = 0.f;
+            if ((rc = time_of(cand, &ms))) return rc;
+            if (ms < best) best = ms, t = cand;
+        }
+        for (int64_t bb : {int64_t(256) << 20, int64_t(128) << 20, int64_t(48) << 20, int64_t(8) << 20}) {
+            if (bb >= static_cast<int64_t>(P.total_states_bytes)) continue;
+            Tune cand = t;
+            cand.block_bytes = bb;
+            float ms = 0.f;
+            if ((rc = time_of(cand, &ms))) return rc;
+            if (ms < best) best = ms, t = cand;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->batch_tuned[key] = {t.grid, t.row_bits, t.block_bytes};
+        tuned = true;
+    }
+
+    // ---- run ----
+    if ((rc = init_states(c, P))) return rc;
+    hipEvent_t ev0, ev1;
+    HIPCALL(hipEventCreate(&ev0));
+    HIPCALL(hipEventCreate(&ev1));
+    int64_t launches = 0;
+    if (p->use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCALL(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        rc = enqueue_run(c, P, p->steps, t, &launches);
+        hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCALL(ce);
+        size_t nodes = 0;
+        HIPCALL(hipGraphGetNodes(graph, nullptr, &nodes));
+        launches = static_cast<int64_t>(nodes);
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(BK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        }
+        HIPCALL(hipEventRecord(ev0, c->stream));
+        hipError_t le = hipGraphLaunch(exec, c->stream);
+        HIPCALL(hipEventRecord(ev1, c->stream));
+        hipError_t se = hipStreamSynchronize(c->stream);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        HIPCALL(le);
+        HIPCALL(se);
+    } else {
+        HIPCALL(hipEventRecord(ev0, c->stream));
+        if ((rc = enqueue_run(c, P, p->steps, t, &launches))) return rc;
+        HIPCALL(hipEventRecord(ev1, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
+    }
+    float ms = 0.f;
+    HIPCALL(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    std::vector<double> res(arr_off * 3);
+    HIPCALL(hipMemcpy(res.data(), P.results, res.size() * 8, hipMemcpyDeviceToHost));
+    for (const Group& g : P.groups)
+        for (size_t j = 0; j < g.idx.size(); ++j) {
+            const size_t a = g.arr_off + j;
+            out[g.idx[j]] = {res[3 * a], res[3 * a + 1], res[3 * a + 2]};
+        }
+    if (stats) {
+        stats->device_ms = ms;
+        stats->grid = t.grid;
+        stats->row_bits = t.row_bits;
+        stats->block_bytes = t.block_bytes;
+        stats->launches = launches;
+        stats->tuned = tuned ? 1 : 0;
+    }
+    return BK_OK;
+}
+
+}  // extern "C"

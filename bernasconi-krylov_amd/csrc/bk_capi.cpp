@@ -23,11 +23,12 @@
 #include <vector>
 
 #include "../../include/bk_capi.h"
+#include "bk_ctx.hpp"
 #include "bk_host.hpp"
 #include "bk_kernels.hpp"
 #include "bk_sweeps.hpp"
 
-namespace {
+namespace bkc {
 
 thread_local std::string g_err;
 
@@ -42,152 +43,11 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HIPCALL(expr)                                                                           \
-    do {                                                                                        \
-        hipError_t _e = (expr);                                                                 \
-        if (_e != hipSuccess)                                                                   \
-            return fail(_e == hipErrorOutOfMemory ? BK_E_NOMEM : BK_E_HIP, "%s failed: %s (%s:%d)", \
-                        #expr, hipGetErrorString(_e), __FILE__, __LINE__);                      \
-    } while (0)
+}  // namespace bkc
 
-size_t amp_bytes(int precision) { return precision == BK_C64 ? 8 : 16; }
+using namespace bkc;
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes)
-    {
-        if (bytes <= cap) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            if (e != hipSuccess) return e;
-            p = nullptr;
-            cap = 0;
-        }
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipSuccess) cap = bytes;
-        return e;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <typename T>
-    T* as() const
-    {
-        return static_cast<T*>(p);
-    }
-};
-
-// HIP-event timer: events are drawn from a pool, recorded around each launch,
-// and resolved into per-name totals when read.
-struct EventTimer : bk::KernelTimer {
-    std::vector<hipEvent_t> pool;
-    size_t used = 0;
-    struct Rec {
-        size_t ev0, ev1;
-        std::string name;
-        double bytes;
-    };
-    std::vector<Rec> recs;
-    size_t open_ev = 0;
-    hipError_t err = hipSuccess;
-    hipEvent_t take()
-    {
-        if (used == pool.size()) {
-            hipEvent_t e = nullptr;
-            hipError_t r = hipEventCreate(&e);
-            if (r != hipSuccess) {
-                err = r;
-                return nullptr;
-            }
-            pool.push_back(e);
-        }
-        return pool[used++];
-    }
-    void begin(hipStream_t s) override
-    {
-        open_ev = used;
-        hipEvent_t e = take();
-        if (e) (void)hipEventRecord(e, s);
-    }
-    void end(hipStream_t s, const char* name, double bytes) override
-    {
-        const size_t i1 = used;
-        hipEvent_t e = take();
-        if (e) (void)hipEventRecord(e, s);
-        if (err == hipSuccess) recs.push_back({open_ev, i1, name, bytes});
-    }
-    void reset()
-    {
-        used = 0;
-        recs.clear();
-        err = hipSuccess;
-    }
-    ~EventTimer() override
-    {
-        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
-    }
-};
-
-}  // namespace
-
-struct bk_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    DevBuf psi;
-    DevBuf psi_alt;  // second state buffer for out-of-place sweeps
-    int n = 0;
-    int precision = 0;
-    bool has_state = false;
-    // The stored state is exp(-i theta_pre E(s)/n) times the true state: the
-    // opening phase of the next Trotter step is applied by the last sweep of
-    // the previous one.  Exported states are corrected first (flush_phase).
-    double theta_pre = 0.0;
-    DevBuf partials;           // 3 doubles per workgroup of the sweep
-    DevBuf series;             // 3 doubles per measurement slot
-    DevBuf count_partials;
-    DevBuf count_out;
-    DevBuf samp_sums, samp_prefix, samp_rand, samp_out;
-    DevBuf blas_partials, blas_out;
-    std::map<std::pair<int, std::pair<int, double>>, DevBuf> phase_tabs;  // (n, (prec, dtp))
-    bk::SweepTuning tuning{};
-    int energy_mode = BK_ENERGY_CACHE16;  // how the LOW sweep gets E(s)
-    bool tuning_pinned = false;           // set by bk_set_tuning: keep across states
-    DevBuf ecache;                        // uint16 E(s) per amplitude (LOW tile layout)
-    int ecache_n = 0;
-    int ecache_prec = 0;
-    bool timing = false;
-    EventTimer timer;
-    // bumped by every call that changes the held state; an adjoint tape is
-    // valid only for the generation it was recorded at
-    uint64_t state_gen = 0;
-    // bumped when the held state is set anew (init / upload): the rounding
-    // carries of the step-phase tables start over with each history
-    uint64_t history_id = 0;
-    struct FbTab {
-        DevBuf master, carry, applied;
-        int count = 0;
-        uint64_t hist = ~0ull;
-    };
-    std::map<std::pair<int, std::pair<int, double>>, FbTab> fb_tabs;  // (n, (prec, dtp))
-    // adjoint tape: the forward run's parameters and its checkpoint states
-    struct Tape {
-        bool valid = false;
-        uint64_t gen = 0;
-        int n = 0, precision = 0, steps = 0, every = 0, nckpt = 0;
-        double dt = 0.0, theta = 0.0;
-        uint64_t s0 = 0;
-        uint32_t e0 = 0;
-        std::vector<double> g;  // field of each step
-        DevBuf ckpt;            // state after every-th step, 2*every-th, ... (< steps)
-    } tape;
-    DevBuf adj_phi, adj_chi, adj_part, adj_grad, adj_drift;
-};
-
-namespace {
+namespace bkc {
 
 int check_ctx(bk_ctx* c)
 {
@@ -245,6 +105,23 @@ int get_phase_tab(bk_ctx* c, int n, int precision, double dtp, const void** out)
     return BK_OK;
 }
 
+std::vector<double> phase_master(int n, double dtp)
+{
+    const int64_t emax = static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6;  // sum_{j<n} j^2
+    int64_t codd = 0;
+    for (int k = 1; k < n; ++k) codd += (n - k) & 1;
+    const int64_t c4 = codd % 4;
+    const size_t cnt = static_cast<size_t>(emax >> 2) + 1;
+    const double w = -(dtp / static_cast<double>(n));
+    std::vector<double> h(2 * cnt);
+    for (size_t m = 0; m < cnt; ++m) {
+        const double th = w * static_cast<double>(4 * static_cast<int64_t>(m) + c4);
+        h[2 * m] = std::cos(th);
+        h[2 * m + 1] = std::sin(th);
+    }
+    return h;
+}
+
 // Step-phase table with error feedback: exp(-i dtp E/n) in FP64 (master), the
 // per-entry rounding carry, and the rounded table the sweeps read.  The carry
 // restarts (all ones) the first time the table is used in a new history.
@@ -261,18 +138,8 @@ int get_fb_tab(bk_ctx* c, int n, int precision, double dtp, bk_ctx::FbTab** out)
             }
             c->fb_tabs.clear();
         }
-        const int64_t emax = static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6;
-        int64_t codd = 0;
-        for (int k = 1; k < n; ++k) codd += (n - k) & 1;
-        const int64_t c4 = codd % 4;
-        const int cnt = static_cast<int>(emax >> 2) + 1;
-        const double w = -(dtp / static_cast<double>(n));
-        std::vector<double> h(2 * static_cast<size_t>(cnt));
-        for (int m = 0; m < cnt; ++m) {
-            const double th = w * static_cast<double>(4 * static_cast<int64_t>(m) + c4);
-            h[2 * m] = std::cos(th);
-            h[2 * m + 1] = std::sin(th);
-        }
+        const std::vector<double> h = phase_master(n, dtp);
+        const int cnt = static_cast<int>(h.size() / 2);
         bk_ctx::FbTab t;
         HIPCALL(t.master.ensure(h.size() * sizeof(double)));
         HIPCALL(t.carry.ensure(h.size() * sizeof(double)));
@@ -326,6 +193,18 @@ int flush_phase(bk_ctx* c)
 bool is_fin(double x) { return std::isfinite(x); }
 
 }  // namespace
+
+// the uint16 E(s) cache of the held state's n, in the LOW-sweep layout of its precision
+int bkc::ensure_ecache(bk_ctx* c)
+{
+    if (c->ecache_n != c->n || c->ecache_prec != c->precision) {
+        HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
+        HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream));
+        c->ecache_n = c->n;
+        c->ecache_prec = c->precision;
+    }
+    return BK_OK;
+}
 
 extern "C" {
 
@@ -385,6 +264,8 @@ void bk_ctx_destroy(bk_ctx* c)
     c->adj_part.release();
     c->adj_grad.release();
     c->adj_drift.release();
+    for (auto& kv : c->batch_bufs) kv.second.release();
+    for (auto& kv : c->batch_ecache) kv.second.release();
     for (auto& kv : c->phase_tabs) kv.second.release();
     for (auto& kv : c->fb_tabs) {
         kv.second.master.release();
@@ -536,17 +417,6 @@ static int prepare_opening(bk_ctx* c, int order, double dt, bool fresh_start, ui
     return BK_OK;
 }
 
-// the uint16 E(s) cache of the held state's n, in the LOW-sweep layout of its precision
-static int ensure_ecache(bk_ctx* c)
-{
-    if (c->ecache_n != c->n || c->ecache_prec != c->precision) {
-        HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
-        HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream));
-        c->ecache_n = c->n;
-        c->ecache_prec = c->precision;
-    }
-    return BK_OK;
-}
 
 // Enqueue one Trotter step of the held state: the rotations, then the phase
 // exp(-i dt E/n) -- for Lie the next step's opening phase, for Strang this

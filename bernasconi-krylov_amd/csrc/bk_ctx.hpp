@@ -1,0 +1,185 @@
+// bk_ctx.hpp -- the library context and the helpers the C ABI files share
+// (private to libbk; not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/bk_capi.h"
+#include "bk_kernels.hpp"
+
+namespace bkc {
+
+// set the thread's last-error message and return code
+int fail(int code, const char* fmt, ...);
+
+#define HIPCALL(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return fail(_e == hipErrorOutOfMemory ? BK_E_NOMEM : BK_E_HIP, "%s failed: %s (%s:%d)", \
+                        #expr, hipGetErrorString(_e), __FILE__, __LINE__);                      \
+    } while (0)
+
+inline size_t amp_bytes(int precision) { return precision == BK_C64 ? 8 : 16; }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+// HIP-event timer: events are drawn from a pool, recorded around each launch,
+// and resolved into per-name totals when read.
+struct EventTimer : bk::KernelTimer {
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    struct Rec {
+        size_t ev0, ev1;
+        std::string name;
+        double bytes;
+    };
+    std::vector<Rec> recs;
+    size_t open_ev = 0;
+    hipError_t err = hipSuccess;
+    hipEvent_t take()
+    {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) {
+                err = r;
+                return nullptr;
+            }
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(hipStream_t s) override
+    {
+        open_ev = used;
+        hipEvent_t e = take();
+        if (e) (void)hipEventRecord(e, s);
+    }
+    void end(hipStream_t s, const char* name, double bytes) override
+    {
+        const size_t i1 = used;
+        hipEvent_t e = take();
+        if (e) (void)hipEventRecord(e, s);
+        if (err == hipSuccess) recs.push_back({open_ev, i1, name, bytes});
+    }
+    void reset()
+    {
+        used = 0;
+        recs.clear();
+        err = hipSuccess;
+    }
+    ~EventTimer() override
+    {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace bkc
+
+struct bk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bkc::DevBuf psi;
+    bkc::DevBuf psi_alt;  // second state buffer for out-of-place sweeps
+    int n = 0;
+    int precision = 0;
+    bool has_state = false;
+    // The stored state is exp(-i theta_pre E(s)/n) times the true state: the
+    // opening phase of the next Trotter step is applied by the last sweep of
+    // the previous one.  Exported states are corrected first (flush_phase).
+    double theta_pre = 0.0;
+    bkc::DevBuf partials;           // 3 doubles per workgroup of the sweep
+    bkc::DevBuf series;             // 3 doubles per measurement slot
+    bkc::DevBuf count_partials;
+    bkc::DevBuf count_out;
+    bkc::DevBuf samp_sums, samp_prefix, samp_rand, samp_out;
+    bkc::DevBuf blas_partials, blas_out;
+    std::map<std::pair<int, std::pair<int, double>>, bkc::DevBuf> phase_tabs;  // (n, (prec, dtp))
+    bk::SweepTuning tuning{};
+    int energy_mode = BK_ENERGY_CACHE16;  // how the LOW sweep gets E(s)
+    bool tuning_pinned = false;           // set by bk_set_tuning: keep across states
+    bkc::DevBuf ecache;                        // uint16 E(s) per amplitude (LOW tile layout)
+    int ecache_n = 0;
+    int ecache_prec = 0;
+    bool timing = false;
+    bkc::EventTimer timer;
+    // bumped by every call that changes the held state; an adjoint tape is
+    // valid only for the generation it was recorded at
+    uint64_t state_gen = 0;
+    // bumped when the held state is set anew (init / upload): the rounding
+    // carries of the step-phase tables start over with each history
+    uint64_t history_id = 0;
+    struct FbTab {
+        bkc::DevBuf master, carry, applied;
+        int count = 0;
+        uint64_t hist = ~0ull;
+    };
+    std::map<std::pair<int, std::pair<int, double>>, FbTab> fb_tabs;  // (n, (prec, dtp))
+    // adjoint tape: the forward run's parameters and its checkpoint states
+    struct Tape {
+        bool valid = false;
+        uint64_t gen = 0;
+        int n = 0, precision = 0, steps = 0, every = 0, nckpt = 0;
+        double dt = 0.0, theta = 0.0;
+        uint64_t s0 = 0;
+        uint32_t e0 = 0;
+        std::vector<double> g;  // field of each step
+        bkc::DevBuf ckpt;            // state after every-th step, 2*every-th, ... (< steps)
+    } tape;
+    bkc::DevBuf adj_phi, adj_chi, adj_part, adj_grad, adj_drift;
+    // batched runs: arena buffers by role, E(s) caches by (n, precision), tuned
+    // parameters by workload shape
+    std::map<std::string, bkc::DevBuf> batch_bufs;
+    std::map<std::pair<int, int>, bkc::DevBuf> batch_ecache;
+    struct BatchTune {
+        int grid = 4096, row_bits = 5;
+        int64_t block_bytes = 0;
+    };
+    std::map<std::string, BatchTune> batch_tuned;
+};
+
+
+namespace bkc {
+
+int check_ctx(bk_ctx* c);
+bool is_fin(double x);
+// exp(-i dtp E/n) for E = 4m + c_n, m = 0..count-1, in FP64 (re, im pairs)
+std::vector<double> phase_master(int n, double dtp);
+// the uint16 E(s) cache of the held state's n (LOW-sweep layout of its precision)
+int ensure_ecache(bk_ctx* c);
+
+}  // namespace bkc

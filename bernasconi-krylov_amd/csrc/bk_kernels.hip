@@ -133,7 +133,11 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // sum over all rotated qubits is Im <chi| sum_q X_q |phi>, unchanged by the
 // rotations because both states receive the same ones and every X_q commutes
 // with them.  One double per workgroup.
-enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32 };
+// F_BATCH: the launch covers many states of the same n stored one after another
+// (state i at psi + i * 2^n); tile t belongs to state t >> inst_shift, whose
+// phase table, butterfly coefficients and P_< / <d_H> references come from
+// per-state arrays.  With F_MEASURE every tile writes its own three partials.
+enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32, F_BATCH = 64 };
 
 struct SweepArgs {
     uint64_t ntiles;
@@ -155,6 +159,13 @@ struct SweepArgs {
     int lo_bits;       // = h - R
     int mid_bits;      // = h2 - (h + a1)
     uint32_t s0, e0;
+    // F_BATCH
+    int inst_shift;             // log2 of the tiles per state
+    int tab_stride;             // phase-table entries per state
+    const double* bc_arr;       // per state (F_TAN: unused)
+    const double* bs_arr;       // per state
+    const uint32_t* s0_arr;     // per state
+    const uint32_t* e0_arr;     // per state
 };
 
 template <int TB_, int RB_, int R_>
@@ -180,6 +191,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     static_assert(NS == 1 || (FLAGS & (F_MEASURE | F_PREFETCH | F_TAN)) == 0,
                   "two-state sweeps use the (c, s) butterfly without measurement or prefetch");
     static_assert((FLAGS & F_GRAD) == 0 || NS == 2, "the gradient pairs two states");
+    constexpr bool BATCH = (FLAGS & F_BATCH) != 0;
+    static_assert(!BATCH || (NS == 1 && (FLAGS & F_PREFETCH) == 0), "batched sweeps: one state per tile, no prefetch");
     constexpr bool LOW = P.low;
     V* sp[NS];
     V* dp[NS];
@@ -297,13 +310,24 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     // header merge two loads in a phi, which the compiler rewrites as a load
     // of a phi of pointers: 2 x NR 64-bit addresses carried around the loop,
     // more than the register file holds beside 2 x NR amplitudes.
-    constexpr bool TOPLOAD = NS == 2;
+    constexpr bool TOPLOAD = NS == 2 || BATCH;
     uint64_t tile = blockIdx.x;
     if constexpr (!TOPLOAD) {
         if (tile < a.ntiles) load(tile_base(tile), x, ec);
     }
     for (; tile < a.ntiles; tile += gridDim.x) {
-        const uint64_t base = tile_base(tile);
+        uint64_t lt = tile;
+        uint32_t inst = 0;
+        R bst = bs, bct = bc;
+        if constexpr (BATCH) {
+            inst = static_cast<uint32_t>(tile >> a.inst_shift);
+            lt = tile & ((1ull << a.inst_shift) - 1ull);
+            sp[0] = psi + (static_cast<uint64_t>(inst) << a.n);
+            dp[0] = sp[0];
+            bst = static_cast<R>(a.bs_arr[inst]);
+            if constexpr ((FLAGS & F_TAN) == 0) bct = static_cast<R>(a.bc_arr[inst]);
+        }
+        const uint64_t base = tile_base(lt);
         const uint64_t nxt = tile + gridDim.x;
         if constexpr (TOPLOAD) load(base, x, ec);
         // software pipelining: the next tile's loads are issued before this
@@ -363,8 +387,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
                     for (int j = 0; j < NR; ++j) {
                         if ((j >> b) & 1) continue;
-                        if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[s][j], x[s][j | (1 << b)], bs);
-                        else bfly_cs<V, R>(x[s][j], x[s][j | (1 << b)], bc, bs);
+                        if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[s][j], x[s][j | (1 << b)], bst);
+                        else bfly_cs<V, R>(x[s][j], x[s][j | (1 << b)], bct, bst);
                     }
             }
         }
@@ -375,6 +399,12 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         if constexpr ((FLAGS & (F_PHASE | F_MEASURE)) != 0) {
             static_assert(LOW, "phase/measure only in the LOW sweep");
             const V* tab = static_cast<const V*>(a.tab);
+            uint32_t e0v = a.e0, s0v = a.s0;
+            if constexpr (BATCH) {
+                tab += static_cast<uint64_t>(inst) * a.tab_stride;
+                e0v = a.e0_arr[inst];
+                s0v = a.s0_arr[inst];
+            }
             const R sc = static_cast<R>(a.scale);
             R tacc[3] = {R(0), R(0), R(0)};
 #pragma unroll
@@ -398,8 +428,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                     // NR terms), folded into FP64 once per tile
                     const R pr = fma(x[0][j].x, x[0][j].x, x[0][j].y * x[0][j].y);
                     tacc[2] += pr;
-                    tacc[0] += (E < a.e0) ? pr : R(0);
-                    tacc[1] = fma(pr, static_cast<R>(__popc(s ^ a.s0)), tacc[1]);
+                    tacc[0] += (E < e0v) ? pr : R(0);
+                    tacc[1] = fma(pr, static_cast<R>(__popc(s ^ s0v)), tacc[1]);
                 }
             }
             if constexpr ((FLAGS & F_MEASURE) != 0) {
@@ -431,6 +461,12 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                 }
             }
         }
+        if constexpr (BATCH && (FLAGS & F_MEASURE) != 0) {
+            // one set of partials per tile (tiles of different states never share a sum)
+            __syncthreads();
+            block_reduce_store<3, NW>(acc, reinterpret_cast<double*>(smem_raw), a.partials + 3 * tile);
+            acc[0] = acc[1] = acc[2] = 0.0;
+        }
         if (nxt < a.ntiles) {
             if constexpr (PF) {
 #pragma unroll
@@ -443,7 +479,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             }
         }
     }
-    if constexpr ((FLAGS & F_MEASURE) != 0) {
+    if constexpr ((FLAGS & F_MEASURE) != 0 && !BATCH) {
         __syncthreads();
         block_reduce_store<3, NW>(acc, reinterpret_cast<double*>(smem_raw), a.partials + 3 * blockIdx.x);
     }
@@ -515,10 +551,9 @@ __global__ __launch_bounds__(kThreads) void k_elementwise(V* __restrict__ psi, u
 }
 
 // sum 'count' triples of partials in a fixed order into out[0..2]
-__global__ __launch_bounds__(kThreads) void k_finish3(const double* __restrict__ partials, int count,
-                                                     double* __restrict__ out)
+__device__ __forceinline__ void finish3_body(const double* __restrict__ partials, int count, double* __restrict__ out,
+                                             double* scratch)
 {
-    __shared__ double scratch[12];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
         acc[0] += partials[3 * i + 0];
@@ -526,6 +561,34 @@ __global__ __launch_bounds__(kThreads) void k_finish3(const double* __restrict__
         acc[2] += partials[3 * i + 2];
     }
     block_reduce_store<3>(acc, scratch, out);
+}
+
+__global__ __launch_bounds__(kThreads) void k_finish3(const double* __restrict__ partials, int count,
+                                                     double* __restrict__ out)
+{
+    __shared__ double scratch[12];
+    finish3_body(partials, count, out, scratch);
+}
+
+// one workgroup per state: its 'count' partial triples into out[3 i .. 3 i + 2]
+__global__ __launch_bounds__(kThreads) void k_finish3_batch(const double* __restrict__ partials, int count,
+                                                           double* __restrict__ out)
+{
+    __shared__ double scratch[12];
+    finish3_body(partials + 3 * static_cast<size_t>(blockIdx.x) * count, count, out + 3 * blockIdx.x, scratch);
+}
+
+// state i of a batch (at psi + i * 2^n) gets amplitude (re_i, im_i) at s0_i
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_set_basis_batch(V* psi, int n, int nstates, const uint32_t* __restrict__ s0,
+                                                             const double* __restrict__ re, const double* __restrict__ im)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nstates; i += gridDim.x * blockDim.x) {
+        V v;
+        v.x = static_cast<typename RealOf<V>::T>(re[i]);
+        v.y = static_cast<typename RealOf<V>::T>(im[i]);
+        psi[(static_cast<uint64_t>(i) << n) + s0[i]] = v;
+    }
 }
 
 template <typename V>
@@ -587,27 +650,49 @@ __global__ __launch_bounds__(kThreads) void k_finish_u64(const unsigned long lon
     if (threadIdx.x == 0) *out = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
 }
 
-// Error-feedback phase table: applied[m] = round(sd * master[m] * carry[m]),
-// carry[m] = exact target / applied (FP64 complex).
+// Error-feedback phase table: applied = round(sd * master * carry),
+// carry <- exact target / applied (FP64 complex).  One entry.
+template <typename V>
+__device__ __forceinline__ void feedback_entry(const double2 w, double2* __restrict__ carry, V* __restrict__ applied,
+                                               double sd)
+{
+    using R = typename RealOf<V>::T;
+    const double2 c = *carry;
+    const double tr = sd * (w.x * c.x - w.y * c.y), ti = sd * (w.x * c.y + w.y * c.x);
+    V a;
+    a.x = static_cast<R>(tr);
+    a.y = static_cast<R>(ti);
+    const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
+    const double inv = 1.0 / (ar * ar + ai * ai);
+    double2 nc;
+    nc.x = (tr * ar + ti * ai) * inv;  // target / a
+    nc.y = (ti * ar - tr * ai) * inv;
+    *carry = nc;
+    *applied = a;
+}
+
 template <typename V>
 __global__ __launch_bounds__(kThreads) void k_phase_feedback(const double2* __restrict__ master,
                                                             double2* __restrict__ carry, V* __restrict__ applied,
                                                             int count, double sd)
 {
-    using R = typename RealOf<V>::T;
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < count; m += gridDim.x * blockDim.x) {
-        const double2 w = master[m], c = carry[m];
-        const double tr = sd * (w.x * c.x - w.y * c.y), ti = sd * (w.x * c.y + w.y * c.x);
-        V a;
-        a.x = static_cast<R>(tr);
-        a.y = static_cast<R>(ti);
-        const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
-        const double inv = 1.0 / (ar * ar + ai * ai);
-        double2 nc;
-        nc.x = (tr * ar + ti * ai) * inv;  // target / a
-        nc.y = (ti * ar - tr * ai) * inv;
-        carry[m] = nc;
-        applied[m] = a;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < count; m += gridDim.x * blockDim.x)
+        feedback_entry<V>(master[m], carry + m, applied + m, sd);
+}
+
+// the same update for nstates tables of 'count' entries (state i: carry/applied
+// entries [i*count, (i+1)*count), scale sd[i]); master is shared
+template <typename V>
+__global__ __launch_bounds__(kThreads) void k_phase_feedback_batch(const double2* __restrict__ master,
+                                                                  double2* __restrict__ carry, V* __restrict__ applied,
+                                                                  int count, int nstates, const double* __restrict__ sd)
+{
+    const int64_t total = static_cast<int64_t>(count) * nstates;
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < total;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int i = static_cast<int>(k / count);
+        const int m = static_cast<int>(k - static_cast<int64_t>(i) * count);
+        feedback_entry<V>(master[m], carry + k, applied + k, sd[i]);
     }
 }
 
@@ -785,6 +870,18 @@ MixCoef mix_coef(int n, double gdt)
     return m;
 }
 
+// norm-exact mixer scale for the coefficients as a kernel of real type Rt sees
+// them: every tan-form butterfly scales the pair norm by 1 + t^2, every (c, s)
+// butterfly by c^2 + s^2
+template <typename Rt>
+double exact_scale(const MixCoef& mc, int n)
+{
+    const double b = static_cast<double>(static_cast<Rt>(mc.bs));
+    if (mc.tan_form) return std::pow(1.0 + b * b, -0.5 * n);
+    const double c = static_cast<double>(static_cast<Rt>(mc.bc));
+    return std::pow(c * c + b * b, -0.5 * n);
+}
+
 template <typename K>
 hipError_t set_lds(K kernel, size_t bytes)
 {
@@ -875,11 +972,7 @@ hipError_t step_impl(V* psi, V* alt, int* in_alt, const StepSpec& sp, const Swee
     if (sp.fb_carry) {
         // norm-exact scale for the coefficients as the kernels see them, folded
         // into the error-feedback table (the sweeps then apply scale 1)
-        const double b = static_cast<double>(static_cast<Rt>(mc.bs));
-        const double sd = mc.tan_form ? std::pow(1.0 + b * b, -0.5 * n)
-                                      : std::pow(static_cast<double>(static_cast<Rt>(mc.bc)) *
-                                                     static_cast<double>(static_cast<Rt>(mc.bc)) + b * b,
-                                                 -0.5 * n);
+        const double sd = exact_scale<Rt>(mc, n);
         const int fg = std::max(1, std::min((sp.fb_count + kThreads - 1) / kThreads, 64));
         hipLaunchKernelGGL(k_phase_feedback<V>, dim3(fg), dim3(kThreads), 0, st,
                            reinterpret_cast<const double2*>(sp.fb_master), reinterpret_cast<double2*>(sp.fb_carry),
@@ -1091,6 +1184,95 @@ hipError_t adj_init_impl(const V* src, V* phi, V* chi, int n, const void* tab, u
     return hipGetLastError();
 }
 
+// ---- batched sweeps: many states of one n in one launch per sweep ----
+template <typename V, int R, bool TAN>
+hipError_t launch_high_batch_r(V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    using PH = HighPlan<C::HTB, C::HRB, R>;
+    auto k = k_sweep<V, PH, F_BATCH | (TAN ? F_TAN : 0), C::SW>;
+    const size_t lds = sizeof(V) << C::HTB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, psi, a);
+    return hipGetLastError();
+}
+
+template <typename V, bool TAN, int R = 3>
+hipError_t launch_high_batch(V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (R > Cfg<V>::HTB - 1) {
+        return hipErrorInvalidValue;
+    } else {
+        if (rbits == R) return launch_high_batch_r<V, R, TAN>(psi, a, grid, st);
+        return launch_high_batch<V, TAN, R + 1>(psi, rbits, a, grid, st);
+    }
+}
+
+template <typename V, int FL>
+hipError_t launch_low_batch(V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW>;
+    const size_t lds = sizeof(V) << C::TB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    return hipGetLastError();
+}
+
+template <typename V>
+hipError_t batch_step_impl(const BatchGroup& g, const SweepTuning& tun, double* partials, hipStream_t st)
+{
+    using C = Cfg<V>;
+    const int n = g.n;
+    if (n < C::TB || !g.ecache) return hipErrorInvalidValue;
+    SweepArgs a{};
+    a.tab = g.tab;
+    a.tab_stride = g.tab_stride;
+    a.ecache = g.ecache;
+    a.scale = 1.0;
+    a.n = n;
+    a.bc_arr = g.bc;
+    a.bs_arr = g.bs;
+    a.s0_arr = g.s0;
+    a.e0_arr = g.e0;
+    a.partials = partials;
+    V* psi = static_cast<V*>(g.psi);
+    const double sweep_bytes = 2.0 * static_cast<double>(sizeof(V)) * static_cast<double>(g.nstates) *
+                               static_cast<double>(1ull << n);
+    hipError_t e = hipSuccess;
+    for (const HighSweep& hs : plan_high_sweeps(n, C::TB, C::HTB, std::max(tun.row_bits, 3), tun.page_bits)) {
+        SweepArgs b = a;
+        b.h = hs.h1;
+        b.a1 = hs.a1;
+        b.h2 = hs.h2;
+        b.lo_bits = hs.h1 - hs.R;
+        b.mid_bits = hs.h2 - (hs.h1 + hs.a1);
+        b.inst_shift = n - C::HTB;
+        b.ntiles = static_cast<uint64_t>(g.nstates) << (n - C::HTB);
+        const int grid = clamp_grid(b.ntiles, tun.grid);
+        if (tun.timer) tun.timer->begin(st);
+        e = g.tan_form ? launch_high_batch<V, true>(psi, hs.R, b, grid, st)
+                       : launch_high_batch<V, false>(psi, hs.R, b, grid, st);
+        if (e != hipSuccess) return e;
+        if (tun.timer) tun.timer->end(st, "batch_sweep_high", sweep_bytes);
+    }
+    a.inst_shift = n - C::TB;
+    a.ntiles = static_cast<uint64_t>(g.nstates) << (n - C::TB);
+    const int grid = clamp_grid(a.ntiles, tun.grid);
+    constexpr int B = F_BATCH | F_PHASE | F_ECACHE;
+    if (tun.timer) tun.timer->begin(st);
+    if (g.tan_form)
+        e = partials ? launch_low_batch<V, B | F_TAN | F_MEASURE>(psi, a, grid, st)
+                     : launch_low_batch<V, B | F_TAN>(psi, a, grid, st);
+    else
+        e = partials ? launch_low_batch<V, B | F_MEASURE>(psi, a, grid, st) : launch_low_batch<V, B>(psi, a, grid, st);
+    if (e != hipSuccess) return e;
+    if (tun.timer) tun.timer->end(st, "batch_sweep_low", sweep_bytes);
+    return hipSuccess;
+}
+
 template <typename V>
 hipError_t phase_from_impl(V* dst, const V* src, int n, const void* tab, double* partials, double* out,
                            hipStream_t st)
@@ -1184,6 +1366,63 @@ hipError_t launch_phase_from(void* dst, const void* src, int n, int precision, c
         return phase_from_impl(static_cast<double2*>(dst), static_cast<const double2*>(src), n, tab, partials, out,
                                stream);
     return hipErrorInvalidValue;
+}
+
+MixInfo mix_info(int n, double gdt, int precision)
+{
+    const MixCoef mc = mix_coef(n, gdt);
+    MixInfo m{};
+    m.tan_form = mc.tan_form;
+    m.bc = mc.bc;
+    m.bs = mc.bs;
+    m.sd = precision == kC64 ? exact_scale<float>(mc, n) : exact_scale<double>(mc, n);
+    return m;
+}
+
+int low_tile_bits(int precision) { return precision == kC64 ? Cfg<float2>::TB : Cfg<double2>::TB; }
+
+hipError_t launch_batch_step(const BatchGroup& g, const SweepTuning& tun, double* partials, hipStream_t stream)
+{
+    if (g.n < 1 || g.n > 32 || g.nstates < 1) return hipErrorInvalidValue;
+    if (g.precision == kC64) return batch_step_impl<float2>(g, tun, partials, stream);
+    if (g.precision == kC128) return batch_step_impl<double2>(g, tun, partials, stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_feedback(const double* master, double* carry, void* applied, int count, int nstates,
+                                 const double* sd, int precision, hipStream_t stream)
+{
+    const int64_t total = static_cast<int64_t>(count) * nstates;
+    const int g = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((total + kThreads - 1) / kThreads, 2048)));
+    if (precision == kC64)
+        hipLaunchKernelGGL(k_phase_feedback_batch<float2>, dim3(g), dim3(kThreads), 0, stream,
+                           reinterpret_cast<const double2*>(master), reinterpret_cast<double2*>(carry),
+                           static_cast<float2*>(applied), count, nstates, sd);
+    else
+        hipLaunchKernelGGL(k_phase_feedback_batch<double2>, dim3(g), dim3(kThreads), 0, stream,
+                           reinterpret_cast<const double2*>(master), reinterpret_cast<double2*>(carry),
+                           static_cast<double2*>(applied), count, nstates, sd);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finish(const double* partials, int tiles_per_state, int nstates, double* out,
+                               hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_finish3_batch, dim3(nstates), dim3(kThreads), 0, stream, partials, tiles_per_state, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_init(void* psi, int n, int nstates, int precision, const uint32_t* s0, const double* re,
+                             const double* im, hipStream_t stream)
+{
+    const int g = std::max(1, std::min((nstates + kThreads - 1) / kThreads, 1024));
+    if (precision == kC64)
+        hipLaunchKernelGGL(k_set_basis_batch<float2>, dim3(g), dim3(kThreads), 0, stream, static_cast<float2*>(psi), n,
+                           nstates, s0, re, im);
+    else
+        hipLaunchKernelGGL(k_set_basis_batch<double2>, dim3(g), dim3(kThreads), 0, stream, static_cast<double2*>(psi),
+                           n, nstates, s0, re, im);
+    return hipGetLastError();
 }
 
 template <typename V>

@@ -89,6 +89,44 @@ hipError_t launch_adj_init(const void* src, void* phi, void* chi, int n, int pre
 hipError_t launch_phase_from(void* dst, const void* src, int n, int precision, const void* tab, double* partials,
                              double* out, hipStream_t stream);
 
+// ---- batched evolution: many states of one n, one launch per sweep ----
+// Mixer coefficients as step_impl uses them, and the norm-exact scale that the
+// error-feedback table folds in (sd).
+struct MixInfo {
+    bool tan_form;
+    double bc, bs, sd;
+};
+MixInfo mix_info(int n, double gdt, int precision);
+int low_tile_bits(int precision);  // TB of the LOW sweep: tiles per state = 2^(n - TB)
+
+struct BatchGroup {
+    int n;
+    int precision;
+    int nstates;
+    void* psi;                // nstates * 2^n amplitudes, state i at i * 2^n
+    const void* tab;          // error-feedback tables, tab_stride entries per state
+    int tab_stride;
+    const uint16_t* ecache;   // E(s) cache of n (shared by the states)
+    const double* bc;         // per state (device)
+    const double* bs;         // per state (device)
+    const uint32_t* s0;       // per state (device)
+    const uint32_t* e0;       // per state (device)
+    bool tan_form;            // every state of the group uses the same butterfly form
+};
+// One Trotter step of every state in g: HIGH sweeps then the LOW sweep with the
+// phase; with partials non-null the LOW sweep also writes 3 partials per tile
+// (finish them with launch_batch_finish).  Same arithmetic as launch_step on
+// each state alone.
+hipError_t launch_batch_step(const BatchGroup& g, const SweepTuning& tun, double* partials, hipStream_t stream);
+// k_phase_feedback for nstates tables (sd per state); master shared
+hipError_t launch_batch_feedback(const double* master, double* carry, void* applied, int count, int nstates,
+                                 const double* sd, int precision, hipStream_t stream);
+hipError_t launch_batch_finish(const double* partials, int tiles_per_state, int nstates, double* out,
+                               hipStream_t stream);
+// state i (zeroed beforehand) gets amplitude re_i + i im_i at s0_i
+hipError_t launch_batch_init(void* psi, int n, int nstates, int precision, const uint32_t* s0, const double* re,
+                             const double* im, hipStream_t stream);
+
 // Read-only measurement of the state into series_slot (3 doubles).
 hipError_t launch_measure(const void* psi, int n, int precision, uint32_t s0, uint32_t e0,
                           double* series_slot, double* partials, int grid, hipStream_t stream);

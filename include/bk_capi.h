@@ -132,6 +132,48 @@ int bk_adjoint_backward(bk_ctx* ctx, double w_pless, double w_dh, double* grad_s
 int bk_adjoint_grad(bk_ctx* ctx, const bk_adjoint_params* p, double w_pless, double w_dh, double* obs3,
                     double* grad_steps, double* drift);
 
+/* ---- batched evolution of many traps (no reference counterpart: the reference
+ * runs one trap per ISampler::sample call, sampler.cuh:412-707) ----
+ * Every instance starts from |start_bits> and takes 'steps' Trotter steps of
+ * its own field g with the common dt / order / precision; out[i] receives
+ * P_<, <d_H>, norm of instance i after the last step (input order).  The
+ * states live in their own device arena (the context's held state is not
+ * touched).  Each instance runs the same kernels and the same arithmetic as a
+ * bk_evolve of that instance alone, so results are bit-identical to serial
+ * runs.  n must be at least the LOW tile (13 for BK_C64, 12 for BK_C128). */
+typedef struct {
+    int n;
+    uint64_t start_bits;
+    int64_t e0;  /* P_< threshold; < 0 -> E(start_bits) */
+    double g;
+} bk_instance;
+
+typedef struct {
+    int precision;  /* BK_C64 | BK_C128 */
+    int order;      /* BK_LIE | BK_STRANG */
+    double dt;
+    int steps;      /* >= 1 */
+    int use_graph;  /* 1: capture the whole run (all steps, all instances) as one hipGraph and launch it once */
+    int autotune;   /* 1: choose grid / row bits / cache-block size by timing short trial runs
+                       (the choice is cached per workload shape on the context) */
+} bk_batch_params;
+
+typedef struct {
+    double p_less, dh, norm;
+} bk_batch_result;
+
+typedef struct {
+    double device_ms;     /* stream time of the evolution (from the first step to the results) */
+    int grid;             /* workgroups per sweep launch */
+    int row_bits;         /* row bits of the high-qubit sweeps */
+    int64_t block_bytes;  /* cache block: states stepped together through all steps (0 = all each step) */
+    int64_t launches;     /* kernel launches of the run (graph nodes with use_graph) */
+    int tuned;            /* 1: the parameters come from the autotuner */
+} bk_batch_stats;
+
+int bk_batch_run(bk_ctx* ctx, const bk_instance* inst, int count, const bk_batch_params* p, bk_batch_result* out,
+                 bk_batch_stats* stats);
+
 /* P_<, <d_H>, norm of the held state (out[3]) */
 int bk_observables(bk_ctx* ctx, uint64_t s0, int64_t e0, double* out3);
 /* exact #{ s : E(s) < e0 } over 2^n states, on the device */

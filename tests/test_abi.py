@@ -173,3 +173,16 @@ def test_host_mt19937_64_and_field_schedule(bk, orc):
     ref = [0.35 + 0.05 * (2.0 * ((c() >> 11) * 2.0 ** -53) - 1.0) for _ in range(64)]
     assert f.tolist() == ref
     assert f.min() >= 0.30 and f.max() <= 0.40
+
+
+def test_batch_workload_shape(bk):
+    """Seed-drawn batch: four sizes chosen by the top two bits of mt19937_64
+    draws, 16 evenly spaced fields, traps from consecutive seeds."""
+    w = bk.batch_workload(64, seed=1)
+    assert len(w) == 64 and {n for n, *_ in w} <= {16, 18, 20, 22}
+    g = bk.MT19937_64(1)
+    assert [n for n, *_ in w] == [(16, 18, 20, 22)[g() >> 62] for _ in range(64)]
+    assert w[0][3] == pytest.approx(0.10) and w[15][3] == pytest.approx(0.60)
+    assert w[16][3] == pytest.approx(0.10)
+    n, s0, e0, _ = w[5]
+    assert bk.make_trap(n, 1005)[1:] == (s0, e0)

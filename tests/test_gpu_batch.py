@@ -1,0 +1,74 @@
+"""GPU tests of the batched evolution (bk_batch_run): every instance must come
+out bit-identical to a bk_evolve of that instance alone (same kernels, same
+arithmetic), with and without graph capture, cache blocking or tuning."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(bk, ns, gs, seed0=40):
+    out = []
+    for i, (n, g) in enumerate(zip(ns, gs)):
+        _, s0, e0 = bk.make_trap(n, seed0 + i)
+        out.append((n, s0, e0, g))
+    return out
+
+
+def _serial(bk, engine, inst, dt, steps, order, prec):
+    rows = []
+    for n, s0, e0, g in inst:
+        r = engine.evolve(n, s0, g, dt, steps, order=order, precision=prec, e0=e0, measure_every=0)
+        rows.append([r["p_less"][-1], r["dh"][-1], r["norm"][-1]])
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("prec,ns", [("c64", [13, 14, 15, 16, 17, 16, 14, 13, 18]),
+                                     ("c128", [12, 13, 14, 15, 13, 12])])
+@pytest.mark.parametrize("order", ["strang", "lie"])
+def test_batch_bit_identical_to_serial(bk, engine, prec, ns, order):
+    precision = bk.BK_C64 if prec == "c64" else bk.BK_C128
+    o = bk.BK_STRANG if order == "strang" else bk.BK_LIE
+    gs = [0.1 + 0.07 * i for i in range(len(ns))]
+    inst = _instances(bk, ns, gs)
+    ref = _serial(bk, engine, inst, 0.02, 37, o, precision)
+    for graph in (False, True):
+        got, st = engine.batch_run(inst, 0.02, 37, order=o, precision=precision, use_graph=graph)
+        assert np.array_equal(got, ref), (graph, np.abs(got - ref).max())
+        assert st["launches"] > 0 and st["device_ms"] > 0
+
+
+def test_batch_mixed_butterfly_forms(bk, engine):
+    """Fields large enough for |cos(g dt)| < 0.5 take the (c, s) butterfly; they
+    form their own launch group and still match serial runs exactly."""
+    ns = [14, 14, 15, 15, 16]
+    gs = [0.35, 70.0, 0.4, 80.0, 65.0]  # g dt = 1.4, 1.6, 1.3 -> (c, s) form
+    inst = _instances(bk, ns, gs, seed0=7)
+    ref = _serial(bk, engine, inst, 0.02, 25, bk.BK_STRANG, bk.BK_C64)
+    got, _ = engine.batch_run(inst, 0.02, 25, order=bk.BK_STRANG, precision=bk.BK_C64)
+    assert np.array_equal(got, ref)
+
+
+def test_batch_tuned_blocks_identical(bk, engine):
+    """The autotuner may pick cache blocks (a block runs all its steps before the
+    next): the per-instance arithmetic, and so the results, do not change."""
+    ns = [16, 18, 16, 18, 17, 17, 16, 18] * 2
+    gs = [0.10 + 0.50 * (i % 16) / 15 for i in range(len(ns))]
+    inst = _instances(bk, ns, gs, seed0=90)
+    ref, _ = engine.batch_run(inst, 0.02, 30, precision=bk.BK_C64, use_graph=False)
+    tuned, st = engine.batch_run(inst, 0.02, 30, precision=bk.BK_C64, use_graph=True, autotune=True)
+    assert st["tuned"]
+    assert np.array_equal(tuned, ref)
+    again, st2 = engine.batch_run(inst, 0.02, 30, precision=bk.BK_C64, use_graph=True, autotune=True)
+    assert st2["tuned"] and (st2["grid"], st2["row_bits"], st2["block_bytes"]) == \
+        (st["grid"], st["row_bits"], st["block_bytes"])
+    assert np.array_equal(again, ref)
+
+
+def test_batch_rejects_small_n(bk, engine):
+    with pytest.raises(bk.BKError) as ei:
+        engine.batch_run([(10, 0, -1, 0.3)], 0.02, 5)
+    assert ei.value.code == -1
+
