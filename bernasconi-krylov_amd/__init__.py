@@ -455,7 +455,8 @@ class Engine:
         _check(lib().bk_set_energy_mode(self._h, mode))
 
     def set_tuning(self, key: str, value: int):
-        """Sweep knobs: 'grid', 'row_bits', 'pingpong'."""
+        """Sweep knobs: 'grid', 'row_bits', 'page_bits', 'pingpong', 'prefetch_low',
+        'prefetch_high', 'fused' (see bk_set_tuning)."""
         _check(lib().bk_set_tuning(self._h, key.encode(), int(value)))
 
     # -- instrumentation

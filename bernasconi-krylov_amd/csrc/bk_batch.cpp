@@ -42,6 +42,8 @@ struct Group {
     size_t part_off = 0;   // partial-triple offset
     int tab_count = 0;     // entries per table
     size_t master_off = 0; // double offset in the master arena
+    std::vector<bk::HighSweep> fused;   // high groups of a fused run (empty: step by step)
+    const uint16_t* ecache_hi = nullptr; // E(s) in the phase-carrying high group's layout
 };
 
 struct Tune {
@@ -102,41 +104,95 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
         b.tan_form = g.tan;
         return b;
     };
-    auto one_step = [&](const Group& g, int first, int count, bool last) -> int {
+    const int tb = bk::low_tile_bits(P.precision);
+    const int htb = tb + 1;  // high-pass tile bits (2^14 c64, 2^13 c128)
+    auto feedback = [&](const Group& g, int first, int count) -> int {
         HIPCALL(bk::launch_batch_feedback(P.master + g.master_off,
                                           P.carry + 2 * (g.tab_off + static_cast<size_t>(first) * g.tab_count),
                                           P.applied + (g.tab_off + static_cast<size_t>(first) * g.tab_count) * P.esz,
                                           g.tab_count, count, P.sd + g.arr_off + first, P.precision, c->stream));
+        ++nl;
+        return BK_OK;
+    };
+    // tiles per state of the pass that measures the last step (the partials' stride)
+    auto final_tiles = [&](const Group& g) -> int {
+        const std::vector<bk::HighSweep>& fg = g.fused;
+        if (fg.empty()) return 1 << (g.n - tb);
+        const int k = steps - 1;
+        const bool low = fg.size() == 2 ? (k & 1) != 0 : (k & 1) == 0;
+        return 1 << (g.n - (low ? tb : htb));
+    };
+    // one step at a time (classic sweeps), or the fused schedule of launch_run
+    auto run_block = [&](const Group& g, int first, int count, int nsteps) -> int {
+        int rc = BK_OK;
         const bk::BatchGroup b = group_block(g, first, count);
-        const size_t tps = static_cast<size_t>(1) << (g.n - bk::low_tile_bits(P.precision));
-        double* part = last ? P.partials + 3 * (g.part_off + static_cast<size_t>(first) * tps) : nullptr;
-        HIPCALL(bk::launch_batch_step(b, tun, part, c->stream));
-        nl += 2 + static_cast<int64_t>(bk::plan_high_sweeps(g.n, bk::low_tile_bits(P.precision),
-                                                            bk::low_tile_bits(P.precision) + 1,
-                                                            std::max(tun.row_bits, 3), tun.page_bits)
-                                           .size());
+        const std::vector<bk::HighSweep>& fg = g.fused;
+        double* part_last = P.partials + 3 * (g.part_off + static_cast<size_t>(first) * final_tiles(g));
+        if (fg.empty()) {
+            for (int k = 0; k < nsteps; ++k) {
+                if ((rc = feedback(g, first, count))) return rc;
+                HIPCALL(bk::launch_batch_step(b, tun, k + 1 == nsteps ? part_last : nullptr, c->stream));
+                nl += 1 + static_cast<int64_t>(
+                              bk::plan_high_sweeps(g.n, tb, htb, std::max(tun.row_bits, 3), tun.page_bits).size());
+            }
+            return BK_OK;
+        }
+        const bool two = fg.size() == 2;
+        bk::BatchPass open{};
+        open.kind = two ? bk::BatchPass::kLowPlain : bk::BatchPass::kHighPlain;
+        open.hs = fg.back();
+        HIPCALL(bk::launch_batch_pass(b, open, tun, nullptr, c->stream));
+        ++nl;
+        for (int k = 0; k < nsteps; ++k) {
+            if (two) {
+                bk::BatchPass pa{};
+                pa.kind = bk::BatchPass::kHighPlain;
+                pa.hs = fg[0];
+                HIPCALL(bk::launch_batch_pass(b, pa, tun, nullptr, c->stream));
+                ++nl;
+            }
+            if ((rc = feedback(g, first, count))) return rc;
+            const bool low = two ? (k & 1) != 0 : (k & 1) == 0;
+            const bool last = k + 1 == nsteps;
+            bk::BatchPass pp{};
+            pp.kind = low ? bk::BatchPass::kLowPhase : bk::BatchPass::kHighPhase;
+            pp.hs = fg.back();
+            pp.ecache_high = g.ecache_hi;
+            pp.measure = last;
+            pp.post = !last;
+            HIPCALL(bk::launch_batch_pass(b, pp, tun, last ? part_last : nullptr, c->stream));
+            ++nl;
+        }
         return BK_OK;
     };
     int rc = BK_OK;
     if (t.block_bytes <= 0) {
-        for (int k = 0; k < steps; ++k)
+        // all states advance together: one pass sequence per group, interleaved by group
+        if (std::all_of(P.groups.begin(), P.groups.end(), [&](const Group& g) { return g.fused.empty(); })) {
+            for (int k = 0; k < steps; ++k)
+                for (const Group& g : P.groups) {
+                    const bk::BatchGroup b = group_block(g, 0, static_cast<int>(g.idx.size()));
+                    if ((rc = feedback(g, 0, static_cast<int>(g.idx.size())))) return rc;
+                    HIPCALL(bk::launch_batch_step(
+                        b, tun, k + 1 == steps ? P.partials + 3 * g.part_off : nullptr, c->stream));
+                    nl += 1 + static_cast<int64_t>(
+                                  bk::plan_high_sweeps(g.n, tb, htb, std::max(tun.row_bits, 3), tun.page_bits).size());
+                }
+        } else {
             for (const Group& g : P.groups)
-                if ((rc = one_step(g, 0, static_cast<int>(g.idx.size()), k + 1 == steps))) return rc;
+                if ((rc = run_block(g, 0, static_cast<int>(g.idx.size()), steps))) return rc;
+        }
     } else {
         for (const Group& g : P.groups) {
             const int64_t sb = static_cast<int64_t>(P.esz) << g.n;
             const int per = static_cast<int>(std::max<int64_t>(1, t.block_bytes / sb));
             const int ns = static_cast<int>(g.idx.size());
-            for (int first = 0; first < ns; first += per) {
-                const int cnt = std::min(per, ns - first);
-                for (int k = 0; k < steps; ++k)
-                    if ((rc = one_step(g, first, cnt, k + 1 == steps))) return rc;
-            }
+            for (int first = 0; first < ns; first += per)
+                if ((rc = run_block(g, first, std::min(per, ns - first), steps))) return rc;
         }
     }
     for (const Group& g : P.groups) {
-        const int tps = 1 << (g.n - bk::low_tile_bits(P.precision));
-        HIPCALL(bk::launch_batch_finish(P.partials + 3 * g.part_off, tps, static_cast<int>(g.idx.size()),
+        HIPCALL(bk::launch_batch_finish(P.partials + 3 * g.part_off, final_tiles(g), static_cast<int>(g.idx.size()),
                                         P.results + 3 * g.arr_off, c->stream));
         ++nl;
     }
@@ -248,13 +304,26 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
     P.partials = static_cast<double*>(v);
     if ((rc = arena(c, "results", arr_off * 3 * 8, &v))) return rc;
     P.results = static_cast<double*>(v);
-    for (const Group& g : P.groups) {
+    for (Group& g : P.groups) {
         DevBuf& ec = c->batch_ecache[{g.n, prec}];
         if (!ec.p) {
             HIPCALL(ec.ensure(sizeof(uint16_t) << g.n));
             HIPCALL(bk::launch_energy_fill(ec.as<uint16_t>(), g.n, prec, c->stream));
         }
         P.ecache[g.n] = ec.as<uint16_t>();
+        // fused schedule (default tuning: the tuner's row bits steer step-by-step sweeps only)
+        std::vector<bk::HighSweep> fg;
+        if (g.tan && c->tuning.fused) fg = bk::fused_sweeps(g.n, prec, bk::default_tuning(g.n, prec));
+        g.fused = fg;
+        if (!fg.empty()) {
+            const bk::HighSweep& h = fg.back();
+            DevBuf& eh = c->batch_ecache_hi[{g.n, prec, h.h1, h.a1, h.h2, h.a2, h.R}];
+            if (!eh.p) {
+                HIPCALL(eh.ensure(sizeof(uint16_t) << g.n));
+                HIPCALL(bk::launch_energy_fill_high(eh.as<uint16_t>(), g.n, prec, h, c->stream));
+            }
+            g.ecache_hi = eh.as<uint16_t>();
+        }
     }
 
     // ---- per-state parameters ----

@@ -266,6 +266,7 @@ void bk_ctx_destroy(bk_ctx* c)
     c->adj_drift.release();
     for (auto& kv : c->batch_bufs) kv.second.release();
     for (auto& kv : c->batch_ecache) kv.second.release();
+    for (auto& kv : c->batch_ecache_hi) kv.second.release();
     for (auto& kv : c->phase_tabs) kv.second.release();
     for (auto& kv : c->fb_tabs) {
         kv.second.master.release();
@@ -422,7 +423,7 @@ static int prepare_opening(bk_ctx* c, int order, double dt, bool fresh_start, ui
 // exp(-i dt E/n) -- for Lie the next step's opening phase, for Strang this
 // step's closing half plus the next step's opening half -- so theta_pre is
 // unchanged.  slot: device pointer for the stepped state's observables, or null.
-static int enqueue_step(bk_ctx* c, double dt, double g, uint64_t s0, uint32_t e0, double* slot)
+static int enqueue_step(bk_ctx* c, double dt, double gdt, uint64_t s0, uint32_t e0, double* slot)
 {
     bk_ctx::FbTab* fb = nullptr;
     int rc = get_fb_tab(c, c->n, c->precision, dt, &fb);
@@ -430,7 +431,7 @@ static int enqueue_step(bk_ctx* c, double dt, double g, uint64_t s0, uint32_t e0
     bk::StepSpec sp{};
     sp.n = c->n;
     sp.precision = c->precision;
-    sp.gdt = g * dt;
+    sp.gdt = gdt;
     sp.dtp = dt;
     sp.phase_tab = fb->applied.p;
     sp.fb_master = fb->master.as<double>();
@@ -468,6 +469,55 @@ static int validate_evolve(const bk_evolve_params* p)
     return BK_OK;
 }
 
+// Enqueue gdt.size() Trotter steps of the held state (g dt of each step in gdt,
+// observables of step k into slots[k] when non-null): a fused run when the plan
+// allows, else step by step.  theta_pre is unchanged.
+static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, const std::vector<double*>& slots,
+                       uint64_t s0, uint32_t e0)
+{
+    const int K = static_cast<int>(gdt.size());
+    if (K == 0) return BK_OK;
+    const std::vector<bk::HighSweep> grp =
+        (c->energy_mode == BK_ENERGY_CACHE16 && !c->tuning.pingpong) ? bk::fused_sweeps(c->n, c->precision, c->tuning)
+                                                                      : std::vector<bk::HighSweep>{};
+    const bool fusable = !grp.empty();
+    if (!fusable) {
+        for (int k = 0; k < K; ++k) {
+            const int rc = enqueue_step(c, dt, gdt[static_cast<size_t>(k)], s0, e0, slots[static_cast<size_t>(k)]);
+            if (rc) return rc;
+        }
+        return BK_OK;
+    }
+    bk_ctx::FbTab* fb = nullptr;
+    int rc = get_fb_tab(c, c->n, c->precision, dt, &fb);
+    if (rc) return rc;
+    if ((rc = ensure_ecache(c))) return rc;
+    const bk::HighSweep& B = grp.back();
+    const std::vector<int> key = {c->n, c->precision, B.h1, B.a1, B.h2, B.a2, B.R};
+    if (c->ecache_hi_key != key) {
+        HIPCALL(c->ecache_hi.ensure(sizeof(uint16_t) << c->n));
+        HIPCALL(bk::launch_energy_fill_high(c->ecache_hi.as<uint16_t>(), c->n, c->precision, B, c->stream));
+        c->ecache_hi_key = key;
+    }
+    bk::RunSpec rs{};
+    rs.n = c->n;
+    rs.precision = c->precision;
+    rs.steps = K;
+    rs.gdt = gdt.data();
+    rs.slots = slots.data();
+    rs.phase_tab = fb->applied.p;
+    rs.fb_master = fb->master.as<double>();
+    rs.fb_carry = fb->carry.as<double>();
+    rs.fb_count = fb->count;
+    rs.s0 = static_cast<uint32_t>(s0);
+    rs.e0 = e0;
+    rs.ecache = c->ecache.as<uint16_t>();
+    rs.ecache_high = c->ecache_hi.as<uint16_t>();
+    HIPCALL(bk::launch_run(c->psi.p, rs, c->tuning, c->partials.as<double>(), c->stream));
+    ++c->state_gen;
+    return BK_OK;
+}
+
 int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
 {
     int rc = check_ctx(c);
@@ -502,11 +552,14 @@ int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
     double* slot0 = (meas[0] == 0) ? ds : nullptr;
     if ((rc = prepare_opening(c, p->order, p->dt, p->init_from_start != 0, s0, e0u, slot0))) return rc;
     if (slot0) slot = 1;
+    std::vector<double> gdt(static_cast<size_t>(K));
+    std::vector<double*> sls(static_cast<size_t>(K), nullptr);
     for (int k = 0; k < K; ++k) {
-        double* sl = (slot < slots && meas[static_cast<size_t>(slot)] == k + 1) ? ds + 3 * (slot++) : nullptr;
-        const double g = p->g_steps ? p->g_steps[k] : p->g;
-        if ((rc = enqueue_step(c, p->dt, g, s0, e0u, sl))) return rc;
+        sls[static_cast<size_t>(k)] =
+            (slot < slots && meas[static_cast<size_t>(slot)] == k + 1) ? ds + 3 * (slot++) : nullptr;
+        gdt[static_cast<size_t>(k)] = (p->g_steps ? p->g_steps[k] : p->g) * p->dt;
     }
+    if ((rc = enqueue_run(c, p->dt, gdt, sls, s0, e0u))) return rc;
     if (out) {
         std::vector<double> h(3 * static_cast<size_t>(slot));
         HIPCALL(hipMemcpyAsync(h.data(), ds, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -567,7 +620,8 @@ int bk_adjoint_forward(bk_ctx* c, const bk_adjoint_params* p, double* obs3)
     char* tape = static_cast<char*>(c->tape.ckpt.p);
     for (int k = 0; k < ev.steps; ++k) {
         const double g = ev.g_steps ? ev.g_steps[k] : ev.g;
-        if ((rc = enqueue_step(c, ev.dt, g, s0, e0u, k + 1 == ev.steps ? c->series.as<double>() : nullptr))) return rc;
+        if ((rc = enqueue_step(c, ev.dt, g * ev.dt, s0, e0u, k + 1 == ev.steps ? c->series.as<double>() : nullptr)))
+            return rc;
         if ((k + 1) % K == 0 && k + 1 < ev.steps)
             HIPCALL(hipMemcpyAsync(tape + bytes * static_cast<size_t>((k + 1) / K - 1), c->psi.p, bytes,
                                    hipMemcpyDeviceToDevice, c->stream));
@@ -991,7 +1045,7 @@ int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits,
         if (step < total) {
             // the step's last sweep reduces the observables of state step+1
             double* slot = (flags[static_cast<size_t>(step) + 1] & 1) ? ds + 3 * (mi++) : nullptr;
-            if ((rc = enqueue_step(c, cfg->dt, cfg->g, start, e0u, slot))) return cleanup_fail(rc);
+            if ((rc = enqueue_step(c, cfg->dt, cfg->g * cfg->dt, start, e0u, slot))) return cleanup_fail(rc);
         }
     }
     std::vector<double> hs(3 * static_cast<size_t>(n_meas));
@@ -1129,6 +1183,8 @@ int bk_set_tuning(bk_ctx* c, const char* key, int value)
     } else if (k == "pingpong") {
         c->tuning.pingpong = value != 0;
         if (!value) c->psi_alt.release();
+    } else if (k == "fused") {
+        c->tuning.fused = value != 0;
     } else {
         return fail(BK_E_INVALID, "unknown tuning key '%s'", key);
     }

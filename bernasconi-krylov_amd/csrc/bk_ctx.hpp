@@ -135,6 +135,8 @@ struct bk_ctx {
     bkc::DevBuf ecache;                        // uint16 E(s) per amplitude (LOW tile layout)
     int ecache_n = 0;
     int ecache_prec = 0;
+    bkc::DevBuf ecache_hi;                     // E(s) in the layout of a fused run's phase-carrying high pass
+    std::vector<int> ecache_hi_key;            // (n, precision, h1, a1, h2, a2, R) it was filled for
     bool timing = false;
     bkc::EventTimer timer;
     // bumped by every call that changes the held state; an adjoint tape is
@@ -165,6 +167,7 @@ struct bk_ctx {
     // parameters by workload shape
     std::map<std::string, bkc::DevBuf> batch_bufs;
     std::map<std::pair<int, int>, bkc::DevBuf> batch_ecache;
+    std::map<std::vector<int>, bkc::DevBuf> batch_ecache_hi;  // (n, precision, h1, a1, h2, a2, R)
     struct BatchTune {
         int grid = 4096, row_bits = 5;
         int64_t block_bytes = 0;
@@ -181,5 +184,6 @@ bool is_fin(double x);
 std::vector<double> phase_master(int n, double dtp);
 // the uint16 E(s) cache of the held state's n (LOW-sweep layout of its precision)
 int ensure_ecache(bk_ctx* c);
+
 
 }  // namespace bkc

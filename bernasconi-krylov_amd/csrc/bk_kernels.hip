@@ -137,7 +137,15 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // (state i at psi + i * 2^n); tile t belongs to state t >> inst_shift, whose
 // phase table, butterfly coefficients and P_< / <d_H> references come from
 // per-state arrays.  With F_MEASURE every tile writes its own three partials.
-enum : int { F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32, F_BATCH = 64 };
+// F_POSTROT (with F_PHASE): after the phase -- the end of one Trotter step --
+// the tile's qubits are rotated once more, with the next step's coefficients
+// (bc2, bs2), walking the phases back to the load layout.  A pass that closes
+// step k then also opens step k+1 on its own qubits, which lets a run cover
+// n <= 32 qubits in two HBM passes per step instead of three or four.
+enum : int {
+    F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32, F_BATCH = 64,
+    F_POSTROT = 128
+};
 
 struct SweepArgs {
     uint64_t ntiles;
@@ -148,6 +156,7 @@ struct SweepArgs {
     void* psi2;              // two-state sweeps: the second state (same tiles, same rotations)
     void* dst2;              // its output (null: in place)
     double bc, bs;     // butterfly coefficients (F_TAN: bs = tan, bc unused)
+    double bc2, bs2;   // F_POSTROT: coefficients of the rotations after the phase
     double scale;      // overall factor applied with the phase (c^n in F_TAN form)
     int n;
     // HIGH: the active local bits [R, TB) map to two global segments,
@@ -192,6 +201,9 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                   "two-state sweeps use the (c, s) butterfly without measurement or prefetch");
     static_assert((FLAGS & F_GRAD) == 0 || NS == 2, "the gradient pairs two states");
     constexpr bool BATCH = (FLAGS & F_BATCH) != 0;
+    constexpr bool POST = (FLAGS & F_POSTROT) != 0;
+    static_assert(!POST || (NS == 1 && (FLAGS & F_PREFETCH) == 0 && (FLAGS & F_PHASE) != 0),
+                  "post-phase rotations: one state, with the phase, no prefetch");
     static_assert(!BATCH || (NS == 1 && (FLAGS & F_PREFETCH) == 0), "batched sweeps: one state per tile, no prefetch");
     constexpr bool LOW = P.low;
     V* sp[NS];
@@ -265,7 +277,13 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         }
     };
     // phase-0 layout loads of one tile (amplitudes and, with the cache, E(s))
-    auto load = [&](uint64_t base, V(&xv)[NS][NR], uint32_t(&ev)[NEC]) {
+    // energy-cache position of a tile: the LOW cache is in state order, tile by
+    // tile; a HIGH pass's cache holds its tiles one after another
+    auto ec_base = [&](uint64_t local_tile, uint64_t base) -> uint64_t {
+        if constexpr (LOW) return base;
+        else return local_tile << TB;
+    };
+    auto load = [&](uint64_t base, uint64_t ecb, V(&xv)[NS][NR], uint32_t(&ev)[NEC]) {
         if constexpr (STRIDED) {
             uint64_t str[RB];
 #pragma unroll
@@ -291,7 +309,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             }
         }
         if constexpr ((FLAGS & F_ECACHE) != 0) {
-            const uint4* ep = reinterpret_cast<const uint4*>(a.ecache + base + static_cast<uint64_t>(t) * NR);
+            const uint4* ep = reinterpret_cast<const uint4*>(a.ecache + ecb + static_cast<uint64_t>(t) * NR);
 #pragma unroll
             for (int q = 0; q < NR / 8; ++q) {
                 const uint4 w = ep[q];
@@ -313,7 +331,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     constexpr bool TOPLOAD = NS == 2 || BATCH;
     uint64_t tile = blockIdx.x;
     if constexpr (!TOPLOAD) {
-        if (tile < a.ntiles) load(tile_base(tile), x, ec);
+        if (tile < a.ntiles) load(tile_base(tile), ec_base(tile, tile_base(tile)), x, ec);
     }
     for (; tile < a.ntiles; tile += gridDim.x) {
         uint64_t lt = tile;
@@ -329,19 +347,19 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         }
         const uint64_t base = tile_base(lt);
         const uint64_t nxt = tile + gridDim.x;
-        if constexpr (TOPLOAD) load(base, x, ec);
+        if constexpr (TOPLOAD) load(base, ec_base(lt, base), x, ec);
         // software pipelining: the next tile's loads are issued before this
         // tile's stores, so waiting for them never waits on the stores
         V xn[NS][PF ? NR : 1];
         uint32_t ecn[PF ? NEC : 1];
         if constexpr (PF) {
-            if (nxt < a.ntiles) load(tile_base(nxt), xn, ecn);
+            if (nxt < a.ntiles) load(tile_base(nxt), ec_base(nxt, tile_base(nxt)), xn, ecn);
         }
         R tg = R(0);
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             if (p > 0) {
-                if constexpr (NS == 2) {
+                if constexpr (NS == 2 || POST) {
                     // swz is linear over XOR and the thread and register bits are
                     // disjoint: swz(tu | pat) = swz(tu) ^ swz(pat).  The thread
                     // part is made opaque per tile so the 2 x NR addresses of
@@ -397,7 +415,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         // next step's phase exp(-i dt E/n), then P_< / <d_H> / norm of the
         // stepped state (|psi|^2 does not see the phase)
         if constexpr ((FLAGS & (F_PHASE | F_MEASURE)) != 0) {
-            static_assert(LOW, "phase/measure only in the LOW sweep");
+            static_assert(LOW || (FLAGS & F_ECACHE) != 0, "a high-qubit pass reads E(s) from its own cache");
             const V* tab = static_cast<const V*>(a.tab);
             uint32_t e0v = a.e0, s0v = a.s0;
             if constexpr (BATCH) {
@@ -407,12 +425,28 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             }
             const R sc = static_cast<R>(a.scale);
             R tacc[3] = {R(0), R(0), R(0)};
+            // d_H weight of register j: s = T | off_j with T = base + tgL (thread and
+            // tile bits) and off_j on the register bits M, disjoint, so
+            // popcount(s ^ s0) = popcount((T ^ s0) & ~M) + popcount(j ^ d), where d
+            // gathers the register-bit positions of T ^ s0 into a register index
+            [[maybe_unused]] int pcT = 0;
+            [[maybe_unused]] uint32_t dj = 0;
+            if constexpr ((FLAGS & F_MEASURE) != 0) {
+                const uint32_t D = static_cast<uint32_t>(base + tgL) ^ s0v;
+                uint32_t M = 0;
+#pragma unroll
+                for (int b = 0; b < RB; ++b) {
+                    const uint32_t gb = static_cast<uint32_t>(gl(1u << P.ph[NP - 1].reg[b]));
+                    M |= gb;
+                    dj |= ((D & gb) ? 1u : 0u) << b;
+                }
+                pcT = __popc(D & ~M);
+            }
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                const uint32_t s = static_cast<uint32_t>(base + tgL + reg_pattern(P.ph[NP - 1], RB, j));
                 uint32_t E;
                 if constexpr ((FLAGS & F_ECACHE) != 0) E = (ec[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-                else E = labs_energy_u32(s, a.n);
+                else E = labs_energy_u32(static_cast<uint32_t>(base + tgL + gl(reg_pattern(P.ph[NP - 1], RB, j))), a.n);
                 if constexpr ((FLAGS & F_PHASE) != 0) {
                     const V ph = tab[E >> 2];  // E = 4m + (n/2 mod 4): one entry per m
                     const R pr = ph.x * sc, pi = ph.y * sc;
@@ -429,12 +463,41 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                     const R pr = fma(x[0][j].x, x[0][j].x, x[0][j].y * x[0][j].y);
                     tacc[2] += pr;
                     tacc[0] += (E < e0v) ? pr : R(0);
-                    tacc[1] = fma(pr, static_cast<R>(__popc(s ^ s0v)), tacc[1]);
+                    tacc[1] = fma(pr, static_cast<R>(pcT + __popc(static_cast<uint32_t>(j) ^ dj)), tacc[1]);
                 }
             }
             if constexpr ((FLAGS & F_MEASURE) != 0) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i) acc[i] += static_cast<double>(tacc[i]);
+            }
+        }
+        if constexpr (POST) {
+            // the next step's rotations of this tile's qubits: phases in
+            // reverse order, ending in the load layout
+            // a batched state keeps its field: the next step's coefficients are its own
+            const R bs2 = BATCH ? bst : static_cast<R>(a.bs2), bc2 = BATCH ? bct : static_cast<R>(a.bc2);
+#pragma unroll
+            for (int p = NP - 1; p >= 0; --p) {
+                if (p < NP - 1) {
+                    uint32_t wb = swz<SW>(tu[p + 1]), rb = swz<SW>(tu[p]);
+                    asm volatile("" : "+v"(wb), "+v"(rb));
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) lds[wb ^ swz<SW>(reg_pattern(P.ph[p + 1], RB, j))] = x[0][j];
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) x[0][j] = lds[rb ^ swz<SW>(reg_pattern(P.ph[p], RB, j))];
+                }
+#pragma unroll
+                for (int b = 0; b < RB; ++b) {
+                    if (!((P.ph[p].act >> b) & 1u)) continue;
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) {
+                        if ((j >> b) & 1) continue;
+                        if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[0][j], x[0][j | (1 << b)], bs2);
+                        else bfly_cs<V, R>(x[0][j], x[0][j | (1 << b)], bc2, bs2);
+                    }
+                }
             }
         }
         if constexpr (STRIDED) {
@@ -454,10 +517,11 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 V* op = dp[s] + base;
+                constexpr int SP = POST ? 0 : NP - 1;  // layout of the registers at the store
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
-                    const uint32_t pj = reg_pattern(P.ph[NP - 1], RB, j);
-                    op[tgL + gl(pj)] = x[s][j];
+                    const uint32_t pj = reg_pattern(P.ph[SP], RB, j);
+                    op[(POST ? tg0 : tgL) + gl(pj)] = x[s][j];
                 }
             }
         }
@@ -475,7 +539,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
                     for (int j = 0; j < NEC; ++j) ec[j] = ecn[j];
             } else if constexpr (!TOPLOAD) {
-                load(tile_base(nxt), x, ec);
+                load(tile_base(nxt), ec_base(nxt, tile_base(nxt)), x, ec);
             }
         }
     }
@@ -618,6 +682,35 @@ __global__ __launch_bounds__(kThreads) void k_energy_fill(uint16_t* __restrict__
         for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
         const uint64_t s = (tile << TB) | u;
         out[i] = static_cast<uint16_t>(labs_energy_u32(static_cast<uint32_t>(s), n));
+    }
+}
+
+// E(s) in the layout in which a HIGH pass (plan PH, geometry h/a1/h2) holds its
+// amplitudes at the phase: tile by tile, thread t's NR values of its last-phase
+// registers contiguous -- the same addressing as k_sweep's tile_base and gl.
+template <class PH>
+__global__ __launch_bounds__(kThreads) void k_energy_fill_high(uint16_t* __restrict__ out, int n, int h, int a1, int h2,
+                                                              int lo_bits, int mid_bits, uint64_t dim)
+{
+    constexpr Plan P = PH::plan;
+    constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R, TT = P.TT();
+    constexpr uint32_t ROWMASK = (1u << RR) - 1u;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t tile = i >> TB;
+        const uint32_t rem = static_cast<uint32_t>(i) & ((1u << TB) - 1u);
+        const uint32_t t = rem >> RB, j = rem & (NR - 1u);
+        uint32_t u = 0;
+        for (int b = 0; b < TT; ++b) u |= ((t >> b) & 1u) << P.ph[NP - 1].thr[b];
+        for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
+        const uint32_t ua = u >> RR;
+        const uint64_t off = static_cast<uint64_t>(u & ROWMASK) + (static_cast<uint64_t>(ua & ((1u << a1) - 1u)) << h) +
+                             (static_cast<uint64_t>(ua >> a1) << h2);
+        const uint64_t lo = tile & ((1ull << lo_bits) - 1ull);
+        const uint64_t mid = (tile >> lo_bits) & ((1ull << mid_bits) - 1ull);
+        const uint64_t hi = tile >> (lo_bits + mid_bits);
+        const uint64_t base = (lo << RR) | (mid << (h + a1)) | (hi << (h2 + (TB - RR - a1)));
+        out[i] = static_cast<uint16_t>(labs_energy_u32(static_cast<uint32_t>(base + off), n));
     }
 }
 
@@ -1273,6 +1366,281 @@ hipError_t batch_step_impl(const BatchGroup& g, const SweepTuning& tun, double* 
     return hipSuccess;
 }
 
+// ---- fused runs: two HBM passes per Trotter step ----
+// row bits of a phase-carrying high pass: 3..12 covers every n whose high
+// qubits fit one or two sweep groups (n <= 32)
+constexpr int kFusedRMin = 3, kFusedRMax = 12;
+
+template <typename V, int FL>
+hipError_t launch_low_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW>;
+    const size_t lds = sizeof(V) << C::TB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    return hipGetLastError();
+}
+
+template <typename V, int R, int FL>
+hipError_t launch_high_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    using PH = HighPlan<C::HTB, C::HRB, R>;
+    auto k = k_sweep<V, PH, FL, C::SW>;
+    const size_t lds = sizeof(V) << C::HTB;
+    hipError_t e = set_lds(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, psi, a);
+    return hipGetLastError();
+}
+
+template <typename V, int FL, int R = kFusedRMin>
+hipError_t launch_highp_r(int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (R > kFusedRMax) {
+        return hipErrorInvalidValue;
+    } else {
+        if (rbits == R) return launch_high_fl<V, R, FL>(psi, a, grid, st);
+        return launch_highp_r<V, FL, R + 1>(rbits, psi, a, grid, st);
+    }
+}
+
+// phase-carrying passes (tan butterflies): F_PHASE | F_ECACHE | F_TAN | any of
+// F_MEASURE, F_POSTROT
+template <typename V>
+hipError_t launch_ppass(bool low, int fl, int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    constexpr int B = F_PHASE | F_ECACHE | F_TAN;
+    switch (fl) {
+    case B: return low ? launch_low_fl<V, B>(psi, a, grid, st) : launch_highp_r<V, B>(rbits, psi, a, grid, st);
+    case B | F_MEASURE:
+        return low ? launch_low_fl<V, B | F_MEASURE>(psi, a, grid, st)
+                   : launch_highp_r<V, B | F_MEASURE>(rbits, psi, a, grid, st);
+    case B | F_POSTROT:
+        return low ? launch_low_fl<V, B | F_POSTROT>(psi, a, grid, st)
+                   : launch_highp_r<V, B | F_POSTROT>(rbits, psi, a, grid, st);
+    case B | F_MEASURE | F_POSTROT:
+        return low ? launch_low_fl<V, B | F_MEASURE | F_POSTROT>(psi, a, grid, st)
+                   : launch_highp_r<V, B | F_MEASURE | F_POSTROT>(rbits, psi, a, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// The high-qubit groups of a fused run: the fewest groups (one or two) that
+// the planner makes with rows of at least 2^kFusedRMin amplitudes, the last
+// group (the phase-carrying one) with R in [kFusedRMin, kFusedRMax].
+template <typename V>
+std::vector<HighSweep> fused_groups(int n, const SweepTuning& tun)
+{
+    using C = Cfg<V>;
+    if (!tun.fused || n <= C::TB) return {};
+    for (int rmin = kFusedRMin + 1; rmin >= kFusedRMin; --rmin) {  // prefer 16-amplitude rows
+        std::vector<HighSweep> plan = plan_high_sweeps(n, C::TB, C::HTB, rmin, tun.page_bits);
+        if (plan.empty() || plan.size() > 2) continue;
+        if (plan.back().R < kFusedRMin || plan.back().R > kFusedRMax) continue;
+        if (plan.size() == 1) return plan;
+        // two groups: both fit with the caller's row preference as well
+        std::vector<HighSweep> pref = plan_high_sweeps(n, C::TB, C::HTB, std::max(tun.row_bits, 3), tun.page_bits);
+        if (pref.size() == 2 && pref.back().R >= kFusedRMin && pref.back().R <= kFusedRMax) return pref;
+        return plan;
+    }
+    return {};
+}
+
+SweepArgs high_args(const SweepArgs& a, const HighSweep& hs, uint64_t dim)
+{
+    SweepArgs b = a;
+    b.h = hs.h1;
+    b.a1 = hs.a1;
+    b.h2 = hs.h2;
+    b.lo_bits = hs.h1 - hs.R;
+    b.mid_bits = hs.h2 - (hs.h1 + hs.a1);
+    b.ntiles = dim >> (hs.R + hs.a1 + hs.a2);
+    return b;
+}
+
+template <typename V>
+hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* partials, hipStream_t st)
+{
+    using C = Cfg<V>;
+    using Rt = typename RealOf<V>::T;
+    const int n = rs.n, K = rs.steps;
+    const uint64_t dim = 1ull << n;
+    hipError_t e = hipSuccess;
+    const std::vector<HighSweep> grp = (K >= 1 && rs.ecache && rs.ecache_high) ? fused_groups<V>(n, tun)
+                                                                               : std::vector<HighSweep>{};
+    bool fused = !grp.empty();
+    for (int k = 0; fused && k < K; ++k)
+        if (!mix_coef(n, rs.gdt[k]).tan_form) fused = false;  // phase-carrying passes use the tan butterfly
+    const bool two = grp.size() == 2;
+    const HighSweep A = fused && two ? grp[0] : HighSweep{};
+    const HighSweep B = fused ? grp.back() : HighSweep{};
+    if (!fused) {
+        for (int k = 0; k < K; ++k) {
+            StepSpec sp{};
+            sp.n = n;
+            sp.precision = rs.precision;
+            sp.gdt = rs.gdt[k];
+            sp.phase_tab = rs.phase_tab;
+            sp.series_slot = rs.slots ? rs.slots[k] : nullptr;
+            sp.s0 = rs.s0;
+            sp.e0 = rs.e0;
+            sp.ecache = rs.ecache;
+            sp.fb_master = rs.fb_master;
+            sp.fb_carry = rs.fb_carry;
+            sp.fb_count = rs.fb_count;
+            int in_alt = 0;
+            if ((e = step_impl(psi, static_cast<V*>(nullptr), &in_alt, sp, tun, partials, st)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    const bool tan = true;
+    SweepArgs a{};
+    a.tab = rs.phase_tab;
+    a.partials = partials;
+    a.scale = 1.0;
+    a.n = n;
+    a.s0 = rs.s0;
+    a.e0 = rs.e0;
+    auto coef = [&](int k, double* bc, double* bs) {
+        const MixCoef m = mix_coef(n, rs.gdt[k]);
+        *bc = m.bc;
+        *bs = m.bs;
+    };
+    const double pass_bytes = 2.0 * static_cast<double>(sizeof(V)) * static_cast<double>(dim);
+    // Two high groups A, B: LOW(1), then per step A(k) and a phase pass that
+    // alternates B (even k) / LOW (odd k).  One high group H: H(1), then per
+    // step a phase pass alternating LOW (even k) / H (odd k) -- one pass a step.
+    if (two) {
+        SweepArgs b = a;
+        coef(0, &b.bc, &b.bs);
+        b.ntiles = dim >> C::TB;
+        const int g = clamp_grid(b.ntiles, tun.grid);
+        if (tun.timer) tun.timer->begin(st);
+        e = launch_low_fl<V, F_TAN>(psi, b, g, st);
+        if (e != hipSuccess) return e;
+        if (tun.timer) tun.timer->end(st, "fused_low_open", pass_bytes);
+    } else {
+        SweepArgs b = high_args(a, B, dim);
+        coef(0, &b.bc, &b.bs);
+        const int g = clamp_grid(b.ntiles, tun.grid);
+        if (tun.timer) tun.timer->begin(st);
+        e = launch_high_dyn<V>(tan, false, psi, B.R, b, g, st);
+        if (e != hipSuccess) return e;
+        if (tun.timer) tun.timer->end(st, "fused_high_open", pass_bytes);
+    }
+    for (int k = 0; k < K; ++k) {
+        // A: the first high-qubit group of step k (two groups only)
+        if (two) {
+            SweepArgs b = high_args(a, A, dim);
+            coef(k, &b.bc, &b.bs);
+            const int g = clamp_grid(b.ntiles, tun.grid);
+            if (tun.timer) tun.timer->begin(st);
+            e = launch_high_dyn<V>(tan, false, psi, A.R, b, g, st);
+            if (e != hipSuccess) return e;
+            if (tun.timer) tun.timer->end(st, "fused_high_a", pass_bytes);
+        }
+        // the step's error-feedback phase table
+        {
+            const MixCoef m = mix_coef(n, rs.gdt[k]);
+            const int fg = std::max(1, std::min((rs.fb_count + kThreads - 1) / kThreads, 64));
+            hipLaunchKernelGGL(k_phase_feedback<V>, dim3(fg), dim3(kThreads), 0, st,
+                               reinterpret_cast<const double2*>(rs.fb_master), reinterpret_cast<double2*>(rs.fb_carry),
+                               static_cast<V*>(const_cast<void*>(rs.phase_tab)), rs.fb_count, exact_scale<Rt>(m, n));
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        // phase-carrying pass
+        const bool low = two ? (k & 1) != 0 : (k & 1) == 0;
+        const bool post = k + 1 < K;
+        double* slot = rs.slots ? rs.slots[k] : nullptr;
+        SweepArgs b = low ? a : high_args(a, B, dim);
+        if (low) b.ntiles = dim >> C::TB;
+        b.ecache = low ? rs.ecache : rs.ecache_high;
+        coef(k, &b.bc, &b.bs);
+        if (post) coef(k + 1, &b.bc2, &b.bs2);
+        const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0);
+        const int g = clamp_grid(b.ntiles, tun.grid);
+        if (tun.timer) tun.timer->begin(st);
+        e = launch_ppass<V>(low, fl, B.R, psi, b, g, st);
+        if (e != hipSuccess) return e;
+        if (tun.timer) tun.timer->end(st, low ? "fused_low_phase" : "fused_high_phase", pass_bytes);
+        if (slot) {
+            hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, g, slot);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+template <typename V, int FL, int R = kFusedRMin>
+hipError_t launch_high_batch_fl(int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (R > kFusedRMax || R >= Cfg<V>::HTB) {
+        return hipErrorInvalidValue;
+    } else {
+        if (rbits == R) return launch_high_fl<V, R, FL>(psi, a, grid, st);
+        return launch_high_batch_fl<V, FL, R + 1>(rbits, psi, a, grid, st);
+    }
+}
+
+template <typename V>
+hipError_t batch_pass_impl(const BatchGroup& g, const BatchPass& bp, const SweepTuning& tun, double* partials,
+                           hipStream_t st)
+{
+    using C = Cfg<V>;
+    const int n = g.n;
+    SweepArgs a{};
+    a.tab = g.tab;
+    a.tab_stride = g.tab_stride;
+    a.scale = 1.0;
+    a.n = n;
+    a.bc_arr = g.bc;
+    a.bs_arr = g.bs;
+    a.s0_arr = g.s0;
+    a.e0_arr = g.e0;
+    a.partials = partials;
+    V* psi = static_cast<V*>(g.psi);
+    const bool low = bp.kind == BatchPass::kLowPlain || bp.kind == BatchPass::kLowPhase;
+    if (low) {
+        a.ecache = g.ecache;
+        a.inst_shift = n - C::TB;
+        a.ntiles = static_cast<uint64_t>(g.nstates) << (n - C::TB);
+    } else {
+        a.h = bp.hs.h1;
+        a.a1 = bp.hs.a1;
+        a.h2 = bp.hs.h2;
+        a.lo_bits = bp.hs.h1 - bp.hs.R;
+        a.mid_bits = bp.hs.h2 - (bp.hs.h1 + bp.hs.a1);
+        a.ecache = bp.ecache_high;
+        a.inst_shift = n - C::HTB;
+        a.ntiles = static_cast<uint64_t>(g.nstates) << (n - C::HTB);
+    }
+    const int grid = clamp_grid(a.ntiles, tun.grid);
+    constexpr int BP = F_BATCH | F_TAN | F_PHASE | F_ECACHE;
+    const int fl = (bp.measure ? F_MEASURE : 0) | (bp.post ? F_POSTROT : 0);
+    switch (bp.kind) {
+    case BatchPass::kLowPlain: return launch_low_batch<V, F_BATCH | F_TAN>(psi, a, grid, st);
+    case BatchPass::kHighPlain: return launch_high_batch<V, true>(psi, bp.hs.R, a, grid, st);
+    case BatchPass::kLowPhase:
+        switch (fl) {
+        case 0: return launch_low_batch<V, BP>(psi, a, grid, st);
+        case F_MEASURE: return launch_low_batch<V, BP | F_MEASURE>(psi, a, grid, st);
+        case F_POSTROT: return launch_low_batch<V, BP | F_POSTROT>(psi, a, grid, st);
+        default: return launch_low_batch<V, BP | F_MEASURE | F_POSTROT>(psi, a, grid, st);
+        }
+    case BatchPass::kHighPhase:
+        switch (fl) {
+        case 0: return launch_high_batch_fl<V, BP>(bp.hs.R, psi, a, grid, st);
+        case F_MEASURE: return launch_high_batch_fl<V, BP | F_MEASURE>(bp.hs.R, psi, a, grid, st);
+        case F_POSTROT: return launch_high_batch_fl<V, BP | F_POSTROT>(bp.hs.R, psi, a, grid, st);
+        default: return launch_high_batch_fl<V, BP | F_MEASURE | F_POSTROT>(bp.hs.R, psi, a, grid, st);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
 template <typename V>
 hipError_t phase_from_impl(V* dst, const V* src, int n, const void* tab, double* partials, double* out,
                            hipStream_t st)
@@ -1368,6 +1736,45 @@ hipError_t launch_phase_from(void* dst, const void* src, int n, int precision, c
     return hipErrorInvalidValue;
 }
 
+std::vector<HighSweep> fused_sweeps(int n, int precision, const SweepTuning& tun)
+{
+    if (precision == kC64) return fused_groups<float2>(n, tun);
+    if (precision == kC128) return fused_groups<double2>(n, tun);
+    return {};
+}
+
+hipError_t launch_run(void* psi, const RunSpec& rs, const SweepTuning& tun, double* partials, hipStream_t stream)
+{
+    if (rs.n < 1 || rs.n > 32 || rs.steps < 0) return hipErrorInvalidValue;
+    if (rs.precision == kC64) return run_impl(static_cast<float2*>(psi), rs, tun, partials, stream);
+    if (rs.precision == kC128) return run_impl(static_cast<double2*>(psi), rs, tun, partials, stream);
+    return hipErrorInvalidValue;
+}
+
+template <class C, int R = kFusedRMin, typename G>
+void fill_high_r(int rbits, G& go)
+{
+    if constexpr (R <= kFusedRMax && R < C::HTB) {
+        if (rbits == R) go(k_energy_fill_high<HighPlan<C::HTB, C::HRB, R>>);
+        else fill_high_r<C, R + 1>(rbits, go);
+    }
+}
+
+hipError_t launch_energy_fill_high(uint16_t* out, int n, int precision, const HighSweep& hs, hipStream_t stream)
+{
+    const uint64_t dim = 1ull << n;
+    const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 65536));
+    const int lo = hs.h1 - hs.R, mid = hs.h2 - (hs.h1 + hs.a1);
+    hipError_t e = hipErrorInvalidValue;
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, out, n, hs.h1, hs.a1, hs.h2, lo, mid, dim);
+        e = hipGetLastError();
+    };
+    if (precision == kC64) fill_high_r<Cfg<float2>>(hs.R, go);
+    else fill_high_r<Cfg<double2>>(hs.R, go);
+    return e;
+}
+
 MixInfo mix_info(int n, double gdt, int precision)
 {
     const MixCoef mc = mix_coef(n, gdt);
@@ -1386,6 +1793,15 @@ hipError_t launch_batch_step(const BatchGroup& g, const SweepTuning& tun, double
     if (g.n < 1 || g.n > 32 || g.nstates < 1) return hipErrorInvalidValue;
     if (g.precision == kC64) return batch_step_impl<float2>(g, tun, partials, stream);
     if (g.precision == kC128) return batch_step_impl<double2>(g, tun, partials, stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_pass(const BatchGroup& g, const BatchPass& bp, const SweepTuning& tun, double* partials,
+                             hipStream_t stream)
+{
+    if (g.n < 1 || g.n > 32 || g.nstates < 1) return hipErrorInvalidValue;
+    if (g.precision == kC64) return batch_pass_impl<float2>(g, bp, tun, partials, stream);
+    if (g.precision == kC128) return batch_pass_impl<double2>(g, bp, tun, partials, stream);
     return hipErrorInvalidValue;
 }
 

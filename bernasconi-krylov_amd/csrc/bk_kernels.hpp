@@ -10,6 +10,8 @@
 
 #include <cstdint>
 
+#include "bk_sweeps.hpp"
+
 namespace bk {
 
 enum Precision : int { kC64 = 1, kC128 = 2 };
@@ -59,6 +61,7 @@ struct SweepTuning {
     bool prefetch_low;   // software-pipelined tile loads in the LOW sweep
     bool prefetch_high;  // ... and in the HIGH sweeps
     int page_bits;       // > 0: deal qubits >= page_bits evenly over the HIGH sweeps
+    bool fused = true;   // runs of steps use the two-pass schedule when the plan allows
     KernelTimer* timer = nullptr;
 };
 
@@ -72,6 +75,39 @@ size_t partials_bytes(int grid);
 // *result_in_alt tells which buffer holds the stepped state.
 hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepSpec& spec,
                        const SweepTuning& tun, double* partials, hipStream_t stream);
+
+// ---- a run of Trotter steps ----
+// A phase-carrying pass rotates its tile's qubits for step k, applies P(k)
+// (and measures), then rotates them again for step k+1; the X_q of one step
+// commute, so only P has to sit between the steps.  With the high qubits in
+// one sweep group H (n <= 24 for complex64) a run of K steps takes K + 1 passes:
+//   H(1), then for k = 1..K a phase pass on LOW (odd k) or H (even k);
+// with two groups A, B (n = 25..32) it takes 2K + 1 passes:
+//   LOW(1), then for k = 1..K: A(k) and a phase pass on B (odd k) / LOW (even k);
+// the last phase pass stops after P(K).  Other n (and the (c, s) butterfly)
+// run step by step (launch_step).  The error-feedback table and the per-step
+// observables are passed as in StepSpec.
+struct RunSpec {
+    int n;
+    int precision;
+    int steps;
+    const double* gdt;       // host: g dt of each step
+    double* const* slots;    // host: device slot for each step's observables (entries may be null; array may be null)
+    const void* phase_tab;   // error-feedback step table (see StepSpec)
+    const double* fb_master;
+    double* fb_carry;
+    int fb_count;
+    uint32_t s0, e0;
+    const uint16_t* ecache;       // LOW-layout E(s) cache
+    const uint16_t* ecache_high;  // E(s) in the phase-carrying high group's layout (launch_energy_fill_high);
+                                  // null: step by step
+};
+hipError_t launch_run(void* psi, const RunSpec& rs, const SweepTuning& tun, double* partials, hipStream_t stream);
+// the high-qubit groups of a fused run for n: one group (one pass a step),
+// two (two passes a step) or none (runs go step by step); the last group is
+// the phase-carrying one
+std::vector<HighSweep> fused_sweeps(int n, int precision, const SweepTuning& tun);
+hipError_t launch_energy_fill_high(uint16_t* out, int n, int precision, const HighSweep& hs, hipStream_t stream);
 
 // ---- reverse pass of the adjoint gradient ----
 // One reverse step of phi (reversed forward state) and chi (adjoint state),
@@ -118,6 +154,18 @@ struct BatchGroup {
 // (finish them with launch_batch_finish).  Same arithmetic as launch_step on
 // each state alone.
 hipError_t launch_batch_step(const BatchGroup& g, const SweepTuning& tun, double* partials, hipStream_t stream);
+// One pass of a fused batched run (the schedule of launch_run, applied to every
+// state of g; a state's field is the same in every step): LOW or high group
+// without the phase, or a phase-carrying pass (tan butterflies) with optional
+// measurement (3 partials per tile) and rotations for the next step.
+struct BatchPass {
+    enum Kind { kLowPlain, kHighPlain, kLowPhase, kHighPhase } kind;
+    HighSweep hs;                  // high passes
+    const uint16_t* ecache_high;   // kHighPhase: E(s) in hs's layout (launch_energy_fill_high)
+    bool measure, post;
+};
+hipError_t launch_batch_pass(const BatchGroup& g, const BatchPass& bp, const SweepTuning& tun, double* partials,
+                             hipStream_t stream);
 // k_phase_feedback for nstates tables (sd per state); master shared
 hipError_t launch_batch_feedback(const double* master, double* carry, void* applied, int count, int nstates,
                                  const double* sd, int precision, hipStream_t stream);

@@ -139,8 +139,10 @@ int bk_adjoint_grad(bk_ctx* ctx, const bk_adjoint_params* p, double w_pless, dou
  * P_<, <d_H>, norm of instance i after the last step (input order).  The
  * states live in their own device arena (the context's held state is not
  * touched).  Each instance runs the same kernels and the same arithmetic as a
- * bk_evolve of that instance alone, so results are bit-identical to serial
- * runs.  n must be at least the LOW tile (13 for BK_C64, 12 for BK_C128). */
+ * step-by-step bk_evolve of that instance alone, so results are bit-identical
+ * to serial runs (for n >= 28 in complex64 compare with the tuning key
+ * "fused" = 0: bk_evolve's two-pass runs round differently).  n must be at
+ * least the LOW tile (13 for BK_C64, 12 for BK_C128). */
 typedef struct {
     int n;
     uint64_t start_bits;
@@ -289,6 +291,10 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  *              sweeps instead of giving them all to the last one (0 = contiguous)
  * "prefetch_low" / "prefetch_high" : 1 = issue the next tile's loads before the
  *              current tile's stores (software pipelining) in that sweep kind
+ * "fused"    : 1 (default) = a run of K steps whose high qubits form two sweep
+ *              groups (n = 28..32 in complex64) takes 2K+1 passes: the pass
+ *              that applies step k's phase also rotates its qubits for step k+1;
+ *              0 = three or more passes per step
  * Values persist for the context until changed. */
 int bk_set_tuning(bk_ctx* ctx, const char* key, int value);
 

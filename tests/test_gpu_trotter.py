@@ -141,27 +141,37 @@ def test_c64_against_c128_n24(engine, bk):
 @pytest.mark.parametrize("n,prec", [(14, C128), (20, C64), (22, C128)])
 def test_energy_modes_bit_identical(engine, orc, bk, n, prec):
     """uint16 energy cache and on-the-fly popcount energies feed the same integer
-    E(s) to the same kernels: the series must agree bit for bit."""
+    E(s) to the same kernels: the series must agree bit for bit.  On-the-fly
+    energies run the three-pass step schedule, so the cached run takes it too."""
     _, s0, e0 = orc.make_trap(n, 31)
-    engine.set_energy_mode(bk.BK_ENERGY_ONTHEFLY)
-    a = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
-    psi_a = engine.download()
-    engine.set_energy_mode(bk.BK_ENERGY_CACHE16)
-    b = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
-    psi_b = engine.download()
+    engine.set_tuning("fused", 0)
+    try:
+        engine.set_energy_mode(bk.BK_ENERGY_ONTHEFLY)
+        a = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
+        psi_a = engine.download()
+        engine.set_energy_mode(bk.BK_ENERGY_CACHE16)
+        b = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
+        psi_b = engine.download()
+    finally:
+        engine.set_energy_mode(bk.BK_ENERGY_CACHE16)
+        engine.set_tuning("fused", 1)
     assert np.array_equal(a["p_less"], b["p_less"]) and np.array_equal(a["dh"], b["dh"])
     assert np.array_equal(psi_a, psi_b)
 
 
+@pytest.mark.parametrize("fused", [0, 1])
 @pytest.mark.parametrize("n,prec", [(17, C128), (22, C64), (25, C64)])
-def test_pingpong_and_grid_bit_identical(engine, orc, bk, n, prec):
+def test_pingpong_and_grid_bit_identical(engine, orc, bk, n, prec, fused):
     """Out-of-place sweeps and other persistent grid sizes change only where
-    data lives, not the arithmetic: results must match the in-place run exactly."""
+    data lives, not the arithmetic: results must match the in-place run exactly.
+    Out-of-place sweeps exist in the step schedule only."""
     _, s0, e0 = orc.make_trap(n, 41)
-    base = engine.evolve(n, s0, 0.35, 0.02, 9, order=STRANG, precision=prec)
-    psi0 = engine.download()
+    engine.set_tuning("fused", fused)
     try:
-        for key, val in (("pingpong", 1), ("grid", 333), ("grid", 100000)):
+        base = engine.evolve(n, s0, 0.35, 0.02, 9, order=STRANG, precision=prec)
+        psi0 = engine.download()
+        knobs = (("grid", 333), ("grid", 100000)) if fused else (("pingpong", 1), ("grid", 333), ("grid", 100000))
+        for key, val in knobs:
             engine.set_tuning(key, val)
             got = engine.evolve(n, s0, 0.35, 0.02, 9, order=STRANG, precision=prec)
             psi = engine.download()
@@ -172,3 +182,36 @@ def test_pingpong_and_grid_bit_identical(engine, orc, bk, n, prec):
     finally:
         engine.set_tuning("pingpong", 0)
         engine.set_tuning("grid", 2048)
+        engine.set_tuning("fused", 1)
+
+
+@pytest.mark.parametrize("n", [14, 17, 19, 22, 25, 28, 30])
+def test_fused_run_matches_step_by_step(bk, n):
+    """Fused runs carry each step's phase inside a pass that also opens the next
+    step on its qubits: one high group takes K+1 passes, two take 2K+1, against
+    3K (or 2K) for the step schedule.  Complex128: equal to the step-by-step run
+    to 1e-12 in every recorded P_<, <d_H>, norm and to 1e-13 in the final
+    state; complex64: within the c64 error of complex128."""
+    _, s0, e0 = bk.make_trap(n, 5)
+    steps, dt, g = 9, 0.02, 0.35
+    out = {}
+    for prec in (bk.BK_C128, bk.BK_C64):
+        for fused in (0, 1):
+            eng = bk.Engine(0)
+            try:
+                eng.set_tuning("fused", fused)
+                r = eng.evolve(n, s0, g, dt, steps, order=bk.BK_STRANG, precision=prec, e0=e0, measure_every=2)
+                psi = eng.download() if prec == bk.BK_C128 and n <= 28 else None
+            finally:
+                eng.close()
+            out[(prec, fused)] = (r, psi)
+    a, b = out[(bk.BK_C128, 0)][0], out[(bk.BK_C128, 1)][0]
+    for key in ("p_less", "dh", "norm"):
+        assert a[key].shape == b[key].shape
+        assert np.abs(a[key] - b[key]).max() <= 1e-12, key
+    if n <= 28:
+        pa, pb = out[(bk.BK_C128, 0)][1], out[(bk.BK_C128, 1)][1]
+        assert np.abs(pa - pb).max() <= 1e-13
+    c = out[(bk.BK_C64, 1)][0]
+    assert np.abs(c["p_less"] - a["p_less"]).max() <= 1e-6
+    assert np.abs(c["dh"] - a["dh"]).max() <= 1e-5
