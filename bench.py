@@ -51,11 +51,8 @@ def _traffic(kernel: str, n: int, precision: str, energy: str):
         return None
     with open(path) as f:
         d = json.load(f)["kernels"]
-    if kernel == "sweep_low":
-        return d["sweep_low"]["traffic_bytes_per_launch"]
-    if kernel.startswith("sweep_high"):
-        return max(v["traffic_bytes_per_launch"] for k, v in d.items() if k.startswith("sweep_high"))
-    return None
+    entry = d.get(kernel)
+    return entry["traffic_bytes_per_launch"] if entry else None
 
 
 def _adjoint_bench(bk, eng, seed: int) -> dict:

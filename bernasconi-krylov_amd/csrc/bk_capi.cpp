@@ -456,6 +456,19 @@ static int enqueue_step(bk_ctx* c, double dt, double gdt, uint64_t s0, uint32_t 
     return BK_OK;
 }
 
+// E(s) of the held n in the layout of the phase-carrying high group hs
+static int ensure_ecache_high(bk_ctx* c, const bk::HighSweep& hs)
+{
+    const std::vector<int> key = {c->n, c->precision, hs.h1, hs.a1, hs.h2, hs.a2, hs.R};
+    if (c->ecache_hi_key != key) {
+        c->ecache_hi_key.clear();
+        HIPCALL(c->ecache_hi.ensure(sizeof(uint16_t) << c->n));
+        HIPCALL(bk::launch_energy_fill_high(c->ecache_hi.as<uint16_t>(), c->n, c->precision, hs, c->stream));
+        c->ecache_hi_key = key;
+    }
+    return BK_OK;
+}
+
 static int validate_evolve(const bk_evolve_params* p)
 {
     if (!p) return fail(BK_E_INVALID, "null params");
@@ -492,13 +505,7 @@ static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, con
     int rc = get_fb_tab(c, c->n, c->precision, dt, &fb);
     if (rc) return rc;
     if ((rc = ensure_ecache(c))) return rc;
-    const bk::HighSweep& B = grp.back();
-    const std::vector<int> key = {c->n, c->precision, B.h1, B.a1, B.h2, B.a2, B.R};
-    if (c->ecache_hi_key != key) {
-        HIPCALL(c->ecache_hi.ensure(sizeof(uint16_t) << c->n));
-        HIPCALL(bk::launch_energy_fill_high(c->ecache_hi.as<uint16_t>(), c->n, c->precision, B, c->stream));
-        c->ecache_hi_key = key;
-    }
+    if ((rc = ensure_ecache_high(c, grp.back()))) return rc;
     bk::RunSpec rs{};
     rs.n = c->n;
     rs.precision = c->precision;
@@ -618,12 +625,18 @@ int bk_adjoint_forward(bk_ctx* c, const bk_adjoint_params* p, double* obs3)
     HIPCALL(c->series.ensure(3 * sizeof(double)));
     if ((rc = prepare_opening(c, ev.order, ev.dt, true, s0, e0u, nullptr))) return rc;
     char* tape = static_cast<char*>(c->tape.ckpt.p);
-    for (int k = 0; k < ev.steps; ++k) {
-        const double g = ev.g_steps ? ev.g_steps[k] : ev.g;
-        if ((rc = enqueue_step(c, ev.dt, g * ev.dt, s0, e0u, k + 1 == ev.steps ? c->series.as<double>() : nullptr)))
-            return rc;
-        if ((k + 1) % K == 0 && k + 1 < ev.steps)
-            HIPCALL(hipMemcpyAsync(tape + bytes * static_cast<size_t>((k + 1) / K - 1), c->psi.p, bytes,
+    // one run per checkpoint interval: a run leaves the state in the plain
+    // frame, which is what the tape holds
+    for (int k = 0; k < ev.steps;) {
+        const int L = std::min(K, ev.steps - k);
+        std::vector<double> gdt(static_cast<size_t>(L));
+        std::vector<double*> slots(static_cast<size_t>(L), nullptr);
+        for (int i = 0; i < L; ++i) gdt[static_cast<size_t>(i)] = (ev.g_steps ? ev.g_steps[k + i] : ev.g) * ev.dt;
+        if (k + L == ev.steps) slots.back() = c->series.as<double>();
+        if ((rc = enqueue_run(c, ev.dt, gdt, slots, s0, e0u))) return rc;
+        k += L;
+        if (k % K == 0 && k < ev.steps)
+            HIPCALL(hipMemcpyAsync(tape + bytes * static_cast<size_t>(k / K - 1), c->psi.p, bytes,
                                    hipMemcpyDeviceToDevice, c->stream));
     }
     double h[3];
@@ -663,20 +676,24 @@ int bk_adjoint_backward(bk_ctx* c, double w_pless, double w_dh, double* grad_ste
     HIPCALL(c->adj_grad.ensure(sizeof(double) * static_cast<size_t>(t.steps)));
     HIPCALL(c->adj_drift.ensure(sizeof(double) * static_cast<size_t>(std::max(t.nckpt, 1))));
     HIPCALL(c->adj_part.ensure(sizeof(double) * std::max<size_t>(4096, bk::reverse_partials_count(n, prec, c->tuning))));
-    bk::StepSpec sp{};
-    sp.n = n;
-    sp.precision = prec;
-    sp.dtp = -t.dt;
+    bk::RevRunSpec rs{};
+    rs.n = n;
+    rs.precision = prec;
     if (n >= low_tb) {
         if ((rc = ensure_ecache(c))) return rc;
-        sp.ecache = c->ecache.as<uint16_t>();
+        rs.ecache = c->ecache.as<uint16_t>();
+        const std::vector<bk::HighSweep> grp = bk::fused_sweeps(n, prec, c->tuning);
+        if (!grp.empty()) {
+            if ((rc = ensure_ecache_high(c, grp.back()))) return rc;
+            rs.ecache_high = c->ecache_hi.as<uint16_t>();
+        }
     }
     const void* tab_init = nullptr;
     const void* tab_rev = nullptr;
     // phi'_T = P(-dt) P(theta_tape - theta_now) * held state
     if ((rc = get_phase_tab(c, n, prec, t.theta - c->theta_pre - t.dt, &tab_init))) return rc;
     if ((rc = get_phase_tab(c, n, prec, -t.dt, &tab_rev))) return rc;
-    sp.phase_tab = tab_rev;
+    rs.phase_tab = tab_rev;
     HIPCALL(bk::launch_adj_init(c->psi.p, c->adj_phi.p, c->adj_chi.p, n, prec, tab_init, static_cast<uint32_t>(t.s0), t.e0,
                                 w_pless, w_dh, c->stream));
     const char* tape = static_cast<const char*>(t.ckpt.p);
@@ -685,14 +702,28 @@ int bk_adjoint_backward(bk_ctx* c, double w_pless, double w_dh, double* grad_ste
     double* part = c->adj_part.as<double>();
     bk::SweepTuning tun = c->tuning;
     tun.timer = c->timing ? &c->timer : nullptr;
-    for (int k = t.steps; k >= 1; --k) {
+    // reverse steps k = steps..1 in runs that end at the checkpoints, where
+    // phi is replaced by the taped state (and the difference recorded)
+    std::vector<double> gdt;
+    std::vector<double*> slots;
+    for (int k = t.steps; k >= 1;) {
         if (k < t.steps && k % t.every == 0) {
             const int i = k / t.every - 1;
             HIPCALL(bk::launch_phase_from(c->adj_phi.p, tape + bytes * static_cast<size_t>(i), n, prec, tab_rev, part,
                                           dd + i, c->stream));
         }
-        sp.gdt = -t.g[static_cast<size_t>(k - 1)] * t.dt;
-        HIPCALL(bk::launch_reverse_step(c->adj_phi.p, c->adj_chi.p, sp, tun, part, dg + (k - 1), c->stream));
+        const int lo = ((k - 1) / t.every) * t.every;
+        gdt.clear();
+        slots.clear();
+        for (int j = k; j > lo; --j) {
+            gdt.push_back(-t.g[static_cast<size_t>(j - 1)] * t.dt);
+            slots.push_back(dg + (j - 1));
+        }
+        rs.steps = k - lo;
+        rs.gdt = gdt.data();
+        rs.grad_slots = slots.data();
+        HIPCALL(bk::launch_reverse_run(c->adj_phi.p, c->adj_chi.p, rs, tun, part, c->stream));
+        k = lo;
     }
     std::vector<double> raw(static_cast<size_t>(t.steps));
     std::vector<double> dr(static_cast<size_t>(std::max(t.nckpt, 1)), 0.0);

@@ -141,7 +141,9 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // the tile's qubits are rotated once more, with the next step's coefficients
 // (bc2, bs2), walking the phases back to the load layout.  A pass that closes
 // step k then also opens step k+1 on its own qubits, which lets a run cover
-// n <= 32 qubits in two HBM passes per step instead of three or four.
+// n <= 32 qubits in two HBM passes per step instead of three or four.  With
+// F_GRAD (two states) the rotations after the phase add their gradient terms
+// to partials2: they belong to the next step.
 enum : int {
     F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32, F_BATCH = 64,
     F_POSTROT = 128
@@ -151,6 +153,7 @@ struct SweepArgs {
     uint64_t ntiles;
     const void* tab;   // phase table (V per energy) for F_PHASE
     double* partials;  // gridDim.x * 3 doubles for F_MEASURE, gridDim.x for F_GRAD
+    double* partials2; // F_GRAD with F_POSTROT: gridDim.x doubles, terms of the post-phase rotations
     const uint16_t* ecache;  // F_ECACHE: E(s) per amplitude (exact, < 2^16)
     void* dst;               // output state (null: in place)
     void* psi2;              // two-state sweeps: the second state (same tiles, same rotations)
@@ -202,8 +205,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     static_assert((FLAGS & F_GRAD) == 0 || NS == 2, "the gradient pairs two states");
     constexpr bool BATCH = (FLAGS & F_BATCH) != 0;
     constexpr bool POST = (FLAGS & F_POSTROT) != 0;
-    static_assert(!POST || (NS == 1 && (FLAGS & F_PREFETCH) == 0 && (FLAGS & F_PHASE) != 0),
-                  "post-phase rotations: one state, with the phase, no prefetch");
+    static_assert(!POST || ((NS == 1 || (FLAGS & F_GRAD) != 0) && (FLAGS & F_PREFETCH) == 0 && (FLAGS & F_PHASE) != 0),
+                  "post-phase rotations: one state (or the gradient's two), with the phase, no prefetch");
     static_assert(!BATCH || (NS == 1 && (FLAGS & F_PREFETCH) == 0), "batched sweeps: one state per tile, no prefetch");
     constexpr bool LOW = P.low;
     V* sp[NS];
@@ -260,6 +263,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     const R bc = static_cast<R>(a.bc), bs = static_cast<R>(a.bs);
     double acc[3] = {0.0, 0.0, 0.0};
     double gacc[1] = {0.0};
+    double gacc2[1] = {0.0};
     constexpr bool PF = (FLAGS & F_PREFETCH) != 0;
     // energy cache words: the cache is stored tile by tile in the last phase's
     // register layout (thread t's NR values contiguous), two uint16 per word
@@ -323,6 +327,21 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 
     V x[NS][NR];
     uint32_t ec[NEC];
+    // F_GRAD: Im conj(chi_j) phi_j' + Im conj(chi_j') phi_j over the pairs of bit b
+    auto grad_terms = [&](int b) -> R {
+        R g = R(0);
+        if constexpr (NS == 2) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                if ((j >> b) & 1) continue;
+                const V c0 = x[1][j], c1 = x[1][j | (1 << b)];
+                const V f0 = x[0][j], f1 = x[0][j | (1 << b)];
+                g += fma(c0.x, f1.y, -c0.y * f1.x);
+                g += fma(c1.x, f0.y, -c1.y * f0.x);
+            }
+        }
+        return g;
+    };
     // Two-state sweeps load each tile at the top of its iteration.  Loading
     // the next tile at the bottom (as the one-state sweeps do) makes the loop
     // header merge two loads in a phi, which the compiler rewrites as a load
@@ -389,17 +408,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
             for (int b = 0; b < RB; ++b) {
                 if (!((P.ph[p].act >> b) & 1u)) continue;
-                if constexpr ((FLAGS & F_GRAD) != 0) {
-                    // Im conj(chi_j) phi_j' + Im conj(chi_j') phi_j over the pairs of bit b
-#pragma unroll
-                    for (int j = 0; j < NR; ++j) {
-                        if ((j >> b) & 1) continue;
-                        const V c0 = x[1][j], c1 = x[1][j | (1 << b)];
-                        const V f0 = x[0][j], f1 = x[0][j | (1 << b)];
-                        tg += fma(c0.x, f1.y, -c0.y * f1.x);
-                        tg += fma(c1.x, f0.y, -c1.y * f0.x);
-                    }
-                }
+                if constexpr ((FLAGS & F_GRAD) != 0) tg += grad_terms(b);
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -476,40 +485,49 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             // reverse order, ending in the load layout
             // a batched state keeps its field: the next step's coefficients are its own
             const R bs2 = BATCH ? bst : static_cast<R>(a.bs2), bc2 = BATCH ? bct : static_cast<R>(a.bc2);
+            R tg2 = R(0);
 #pragma unroll
             for (int p = NP - 1; p >= 0; --p) {
                 if (p < NP - 1) {
                     uint32_t wb = swz<SW>(tu[p + 1]), rb = swz<SW>(tu[p]);
                     asm volatile("" : "+v"(wb), "+v"(rb));
-                    __syncthreads();
 #pragma unroll
-                    for (int j = 0; j < NR; ++j) lds[wb ^ swz<SW>(reg_pattern(P.ph[p + 1], RB, j))] = x[0][j];
-                    __syncthreads();
+                    for (int s = 0; s < NS; ++s) {
+                        __syncthreads();
 #pragma unroll
-                    for (int j = 0; j < NR; ++j) x[0][j] = lds[rb ^ swz<SW>(reg_pattern(P.ph[p], RB, j))];
+                        for (int j = 0; j < NR; ++j) lds[wb ^ swz<SW>(reg_pattern(P.ph[p + 1], RB, j))] = x[s][j];
+                        __syncthreads();
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) x[s][j] = lds[rb ^ swz<SW>(reg_pattern(P.ph[p], RB, j))];
+                    }
                 }
 #pragma unroll
                 for (int b = 0; b < RB; ++b) {
                     if (!((P.ph[p].act >> b) & 1u)) continue;
+                    if constexpr ((FLAGS & F_GRAD) != 0) tg2 += grad_terms(b);
 #pragma unroll
-                    for (int j = 0; j < NR; ++j) {
-                        if ((j >> b) & 1) continue;
-                        if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[0][j], x[0][j | (1 << b)], bs2);
-                        else bfly_cs<V, R>(x[0][j], x[0][j | (1 << b)], bc2, bs2);
-                    }
+                    for (int s = 0; s < NS; ++s)
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) {
+                            if ((j >> b) & 1) continue;
+                            if constexpr ((FLAGS & F_TAN) != 0) bfly_tan<V, R>(x[s][j], x[s][j | (1 << b)], bs2);
+                            else bfly_cs<V, R>(x[s][j], x[s][j | (1 << b)], bc2, bs2);
+                        }
                 }
             }
+            if constexpr ((FLAGS & F_GRAD) != 0) gacc2[0] += static_cast<double>(tg2);
         }
         if constexpr (STRIDED) {
+            // registers are in the last phase's layout, or (POST) in the load layout
             uint64_t str[RB];
 #pragma unroll
             for (int b = 0; b < RB; ++b) {
-                str[b] = sstr[b];
+                str[b] = POST ? lstr[b] : sstr[b];
                 asm volatile("" : "+v"(str[b]));
             }
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                V* op = dp[s] + base + tgL;
+                V* op = dp[s] + base + (POST ? tg0 : tgL);
 #pragma unroll
                 for (int j = 0; j < NR; ++j) op[reg_off(str, j)] = x[s][j];
             }
@@ -550,6 +568,10 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     if constexpr ((FLAGS & F_GRAD) != 0) {
         __syncthreads();
         block_reduce_store<1, NW>(gacc, reinterpret_cast<double*>(smem_raw), a.partials + blockIdx.x);
+        if constexpr (POST) {
+            __syncthreads();  // thread 0 has read the scratch words before they are reused
+            block_reduce_store<1, NW>(gacc2, reinterpret_cast<double*>(smem_raw), a.partials2 + blockIdx.x);
+        }
     }
 }
 
@@ -1186,11 +1208,11 @@ hipError_t launch_high_rev(V* phi, int rbits, const SweepArgs& a, int grid, hipS
     }
 }
 
-template <typename V>
+template <typename V, int FL = F_PHASE | F_GRAD | F_ECACHE>
 hipError_t launch_low_rev(V* phi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, F_PHASE | F_GRAD | F_ECACHE, C::SW, 2>;
+    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW, 2>;
     const size_t lds = sizeof(V) << C::TB;
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
@@ -1574,6 +1596,124 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
     return hipSuccess;
 }
 
+// ---- fused reverse runs (adjoint): the schedule of run_impl on two states ----
+constexpr int kRevPass = F_PHASE | F_GRAD | F_ECACHE | F_POSTROT;
+
+template <typename V, int R = kFusedRMin>
+hipError_t launch_high_rev_ppass(V* phi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+{
+    using C = Cfg<V>;
+    if constexpr (R > kFusedRMax || R >= C::HTB) {
+        return hipErrorInvalidValue;
+    } else {
+        if (rbits != R) return launch_high_rev_ppass<V, R + 1>(phi, rbits, a, grid, st);
+        using PH = HighPlan<C::HTB, C::HRB, R>;
+        auto k = k_sweep<V, PH, kRevPass, C::SW, 2>;
+        const size_t lds = sizeof(V) << C::HTB;
+        hipError_t e = set_lds(k, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, phi, a);
+        return hipGetLastError();
+    }
+}
+
+// Reverse steps i = 0..K-1 of phi and chi, each the rotations exp(i gdt_i X)
+// then the phase from rs.phase_tab; step i's gradient sum goes to
+// grad_slots[i].  Two high groups A, B: LOW(0), then per step A(i) and a
+// phase pass alternating B (even i) / LOW (odd i); one group H: H(0), then a
+// phase pass alternating LOW (even i) / H (odd i).  A phase pass adds the terms
+// of its first rotations to step i and those after the phase to step i + 1;
+// the last pass rotates by the identity (c = 1, s = 0, exact) after its phase.
+// Every qubit group of step i has written its partials into region i & 1 when
+// step i's phase pass ends, which is when the region is summed.
+template <typename V>
+hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTuning& tun, double* gpart,
+                            hipStream_t st)
+{
+    using C = Cfg<V>;
+    const int n = rs.n, K = rs.steps;
+    const uint64_t dim = 1ull << n;
+    hipError_t e = hipSuccess;
+    const std::vector<HighSweep> grp = (K >= 1 && rs.ecache && rs.ecache_high) ? fused_groups<V>(n, tun)
+                                                                               : std::vector<HighSweep>{};
+    if (grp.empty()) {
+        for (int i = 0; i < K; ++i) {
+            StepSpec sp{};
+            sp.n = n;
+            sp.precision = rs.precision;
+            sp.gdt = rs.gdt[i];
+            sp.phase_tab = rs.phase_tab;
+            sp.ecache = rs.ecache;
+            if ((e = reverse_step_impl(phi, chi, sp, tun, gpart, rs.grad_slots[i], st)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    const bool two = grp.size() == 2;
+    const HighSweep A = two ? grp[0] : HighSweep{};
+    const HighSweep B = grp.back();
+    SweepArgs a{};
+    a.tab = rs.phase_tab;
+    a.scale = 1.0;
+    a.n = n;
+    a.psi2 = chi;
+    const int gL = clamp_grid(dim >> C::TB, tun.grid), gH = clamp_grid(dim >> C::HTB, tun.grid);
+    const int offA = gL, offB = gL + (two ? gH : 0), GS = offB + gH;
+    auto region = [&](int i) { return gpart + static_cast<size_t>(i & 1) * static_cast<size_t>(GS); };
+    auto coef = [&](int i, double* bc, double* bs) {
+        *bc = std::cos(rs.gdt[i]);
+        *bs = std::sin(rs.gdt[i]);
+    };
+    const double pass_bytes = 4.0 * static_cast<double>(sizeof(V)) * static_cast<double>(dim);
+    auto timed = [&](const char* name, auto&& launch) -> hipError_t {
+        if (tun.timer) tun.timer->begin(st);
+        const hipError_t r = launch();
+        if (r == hipSuccess && tun.timer) tun.timer->end(st, name, pass_bytes);
+        return r;
+    };
+    if (two) {
+        SweepArgs b = a;
+        b.ntiles = dim >> C::TB;
+        coef(0, &b.bc, &b.bs);
+        b.partials = region(0);
+        e = timed("adjoint_open", [&] { return launch_low_rev<V, F_GRAD>(phi, b, gL, st); });
+    } else {
+        SweepArgs b = high_args(a, B, dim);
+        coef(0, &b.bc, &b.bs);
+        b.partials = region(0) + offB;
+        e = timed("adjoint_open", [&] { return launch_high_rev<V>(phi, B.R, b, gH, st); });
+    }
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < K; ++i) {
+        if (two) {
+            SweepArgs b = high_args(a, A, dim);
+            coef(i, &b.bc, &b.bs);
+            b.partials = region(i) + offA;
+            if ((e = timed("adjoint_high_a", [&] { return launch_high_rev<V>(phi, A.R, b, gH, st); })) != hipSuccess)
+                return e;
+        }
+        const bool low = two ? (i & 1) != 0 : (i & 1) == 0;
+        SweepArgs b = low ? a : high_args(a, B, dim);
+        if (low) b.ntiles = dim >> C::TB;
+        b.ecache = low ? rs.ecache : rs.ecache_high;
+        coef(i, &b.bc, &b.bs);
+        if (i + 1 < K) {
+            coef(i + 1, &b.bc2, &b.bs2);
+        } else {
+            b.bc2 = 1.0;
+            b.bs2 = 0.0;
+        }
+        const int off = low ? 0 : offB;
+        b.partials = region(i) + off;
+        b.partials2 = region(i + 1) + off;
+        e = low ? timed("adjoint_low_phase", [&] { return launch_low_rev<V, kRevPass>(phi, b, gL, st); })
+                : timed("adjoint_high_phase", [&] { return launch_high_rev_ppass<V>(phi, B.R, b, gH, st); });
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, region(i), GS, rs.grad_slots[i]);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 template <typename V, int FL, int R = kFusedRMin>
 hipError_t launch_high_batch_fl(int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
@@ -1696,7 +1836,21 @@ size_t reverse_partials_count(int n, int precision, const SweepTuning& tun)
     size_t total = static_cast<size_t>(clamp_grid(dim >> tb, tun.grid));
     for (size_t i = 0; i < plan_high_sweeps(n, tb, htb, std::max(tun.row_bits, 3), tun.page_bits).size(); ++i)
         total += static_cast<size_t>(clamp_grid(dim >> htb, tun.grid));
-    return total;
+    // fused reverse runs: two regions of LOW + two high groups
+    const size_t fused = 2 * (static_cast<size_t>(clamp_grid(dim >> tb, tun.grid)) +
+                              2 * static_cast<size_t>(clamp_grid(dim >> htb, tun.grid)));
+    return std::max(total, fused);
+}
+
+hipError_t launch_reverse_run(void* phi, void* chi, const RevRunSpec& rs, const SweepTuning& tun, double* gpart,
+                              hipStream_t stream)
+{
+    if (rs.n < 1 || rs.n > 32 || rs.steps < 0) return hipErrorInvalidValue;
+    if (rs.precision == kC64)
+        return reverse_run_impl(static_cast<float2*>(phi), static_cast<float2*>(chi), rs, tun, gpart, stream);
+    if (rs.precision == kC128)
+        return reverse_run_impl(static_cast<double2*>(phi), static_cast<double2*>(chi), rs, tun, gpart, stream);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_reverse_step(void* phi, void* chi, const StepSpec& spec, const SweepTuning& tun, double* gpart,

@@ -118,6 +118,24 @@ hipError_t launch_energy_fill_high(uint16_t* out, int n, int precision, const Hi
 hipError_t launch_reverse_step(void* phi, void* chi, const StepSpec& spec, const SweepTuning& tun, double* gpart,
                                double* grad_slot, hipStream_t stream);
 size_t reverse_partials_count(int n, int precision, const SweepTuning& tun);
+// K reverse steps in one fused run (the two-state form of launch_run's
+// schedule, K + 1 or 2K + 1 passes): step i rotates by exp(i gdt[i] X), then
+// applies phase_tab; *grad_slots[i] <- Im <chi| sum_q X_q |phi> as step i
+// begins.  The states leave the run in the plain frame, so a run can stop at a
+// checkpoint.  Without ecache_high, or for n that launch_run also takes step
+// by step, the steps run one by one (launch_reverse_step).
+struct RevRunSpec {
+    int n;
+    int precision;
+    int steps;
+    const double* gdt;           // host: per reverse step
+    double* const* grad_slots;   // host: device slot per reverse step
+    const void* phase_tab;       // exp(-i dtp E/n) per E >> 2
+    const uint16_t* ecache;      // LOW-layout E(s)
+    const uint16_t* ecache_high; // E(s) in the phase-carrying high group's layout
+};
+hipError_t launch_reverse_run(void* phi, void* chi, const RevRunSpec& rs, const SweepTuning& tun, double* gpart,
+                              hipStream_t stream);
 // phi = tab[E] * src; chi = (wp [E < e0] + wd popcount(s ^ s0)) * phi
 hipError_t launch_adj_init(const void* src, void* phi, void* chi, int n, int precision, const void* tab, uint32_t s0,
                            uint32_t e0, double wp, double wd, hipStream_t stream);

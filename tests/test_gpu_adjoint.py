@@ -99,6 +99,41 @@ def test_adjoint_matches_fd_of_gpu_forward(bk, engine):
         assert gseg[j] == pytest.approx(fd, abs=1e-4), (j, gseg[j], fd)
 
 
+@pytest.mark.parametrize("n,prec", [(17, "c128"), (22, "c128"), (23, "c64"), (24, "c64"), (25, "c64")])
+def test_fused_reverse_matches_step_by_step(bk, n, prec):
+    """Fused reverse runs (the forward run's schedule on both states; the
+    gradient terms of the rotations after a phase belong to the next step;
+    runs end at the checkpoints) against the step-by-step reverse pass, which
+    the oracle pins above.  One high group: n = 17 (c128), 23 (c64); two: the
+    rest.  50 steps with checkpoints every 16 give runs of 2, 16, 16, 16 steps.
+    complex128: 1e-10 of the largest component; complex64: 1e-4 (the two
+    orders of FP32 rounding, each within 5e-5 of FP64)."""
+    precision = bk.BK_C128 if prec == "c128" else bk.BK_C64
+    _, s0, e0 = bk.make_trap(n, 7)
+    gs = np.repeat(_fields(5, 40 + n), 10)
+    out = {}
+    for fused in (0, 1):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("fused", fused)
+            obs = eng.adjoint_forward(n, s0, gs, 0.02, order=bk.BK_STRANG, precision=precision, e0=e0,
+                                      checkpoint_every=16)
+            grad, drift = eng.adjoint_backward(1.0, 0.25)
+        finally:
+            eng.close()
+        out[fused] = (np.asarray(obs), grad, drift)
+    (o0, g0, d0), (o1, g1, d1) = out[0], out[1]
+    scale = np.abs(g0).max()
+    tol = 1e-10 if prec == "c128" else 1e-4
+    print(f"\n[fused reverse n={n} {prec}] max|dg| {np.abs(g1 - g0).max():.3e} of {scale:.3e}, "
+          f"drift {d0:.2e} / {d1:.2e}")
+    # P_<, <d_H> (O(n)), norm of the forward runs
+    otol = np.array([1e-12, 1e-11, 1e-12]) if prec == "c128" else np.array([1e-6, 1e-5, 1e-6])
+    assert np.all(np.abs(o1 - o0) <= otol), np.abs(o1 - o0)
+    assert np.abs(g1 - g0).max() <= tol * scale + 1e-12
+    assert d1 < (1e-20 if prec == "c128" else 1e-9)
+
+
 def test_backward_leaves_state_and_detects_stale_tape(bk, engine):
     n = 13
     _, s0, e0 = bk.make_trap(n, 1)
