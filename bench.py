@@ -108,7 +108,9 @@ def _batch_bench(bk, eng, seed: int) -> dict:
     16 fields evenly spaced in [0.10, 0.60], Strang dt=0.02 x 500 steps, c64,
     the whole run one hipGraph launch with autotuned sweep parameters.  The
     serial baseline runs 8 instances of each n one bk_evolve at a time (same
-    kernels, observables at the end) and scales by the instance counts."""
+    kernels, observables at the end) and scales by the instance counts; each
+    n's instances also run as a batch of their own (per-n ratio and HBM rate,
+    counting 18 B per amplitude and step: state read and written, uint16 E)."""
     import numpy as np
     steps, dt = 500, 0.02
     inst = bk.batch_workload(512, seed=seed)
@@ -128,7 +130,18 @@ def _batch_bench(bk, eng, seed: int) -> dict:
             identical &= bool(r["p_less"][-1] == res[i, 0] and r["dh"][-1] == res[i, 1])
         per = (time.perf_counter() - t0) / len(pick)
         serial_est += per * len(idx)
-        sample[str(n)] = {"instances": len(idx), "serial_s_per_instance": per}
+        # the same instances of this n as a batch of their own
+        sub = [inst[i] for i in idx]
+        eng.batch_run(sub, dt, steps, use_graph=True, autotune=True)
+        t0 = time.perf_counter()
+        rs, _ = eng.batch_run(sub, dt, steps, use_graph=True, autotune=True)
+        tb = time.perf_counter() - t0
+        identical &= bool(np.array_equal(rs, res[idx]))
+        step_bytes = len(idx) * float(1 << n) * 18.0  # state read + written (c64) + uint16 energy
+        sample[str(n)] = {"instances": len(idx), "serial_s_per_instance": per,
+                          "serial_s_estimated": per * len(idx), "batched_s": tb,
+                          "batched_over_serial": per * len(idx) / tb,
+                          "batched_hbm_gbs": step_bytes * steps / tb / 1e9}
     return {
         "workload": f"512 traps, n in {{16,18,20,22}} (mt19937_64 seed {seed}), g = 0.10..0.60 (16 values), "
                     f"Strang dt={dt} x {steps} steps, c64, one hipGraph launch",
@@ -138,7 +151,7 @@ def _batch_bench(bk, eng, seed: int) -> dict:
         "serial_s_estimated": serial_est,
         "serial_amp_updates_per_s": amps / serial_est,
         "batched_over_serial": serial_est / t_batch,
-        "serial_sample": sample,
+        "per_n": sample,
         "bit_identical_to_serial_sample": identical,
         "tuning": {"grid": st["grid"], "row_bits": st["row_bits"], "block_bytes": st["block_bytes"]},
         "graph_nodes": st["launches"],

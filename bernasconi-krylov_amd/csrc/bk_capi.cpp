@@ -1061,10 +1061,22 @@ int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits,
     double* slot0 = (flags[0] & 1) ? ds : nullptr;
     if ((rc = prepare_opening(c, order, cfg->dt, true, start, e0u, slot0))) return cleanup_fail(rc);
     if (slot0) mi = 1;
+    // the steps between two sampling points go as one run (a run ends in the
+    // plain frame, whose |psi|^2 the sampler reads)
+    std::vector<double> run_gdt;
+    std::vector<double*> run_slots;
+    auto flush_run = [&]() -> int {
+        if (run_gdt.empty()) return BK_OK;
+        const int e = enqueue_run(c, cfg->dt, run_gdt, run_slots, start, e0u);
+        run_gdt.clear();
+        run_slots.clear();
+        return e;
+    };
     for (int step = 0; step <= total; ++step) {
         const unsigned char f = flags[static_cast<size_t>(step)];
         if (f & 1) tp.push_back(step * cfg->dt);
         if (f & 2) {  // sampling reads |psi|^2 of state 'step' (before this step's evolution)
+            if ((rc = flush_run())) return cleanup_fail(rc);
             for (int s = 0; s < n_seeds; ++s) {
                 const size_t off = pi * per_point + static_cast<size_t>(s) * shots;
                 HIPCALL(bk::launch_sample(c->psi.p, n, precision, c->samp_rand.as<double>() + off, shots,
@@ -1075,10 +1087,11 @@ int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits,
         }
         if (step < total) {
             // the step's last sweep reduces the observables of state step+1
-            double* slot = (flags[static_cast<size_t>(step) + 1] & 1) ? ds + 3 * (mi++) : nullptr;
-            if ((rc = enqueue_step(c, cfg->dt, cfg->g * cfg->dt, start, e0u, slot))) return cleanup_fail(rc);
+            run_slots.push_back((flags[static_cast<size_t>(step) + 1] & 1) ? ds + 3 * (mi++) : nullptr);
+            run_gdt.push_back(cfg->g * cfg->dt);
         }
     }
+    if ((rc = flush_run())) return cleanup_fail(rc);
     std::vector<double> hs(3 * static_cast<size_t>(n_meas));
     if (n_meas > 0)
         HIPCALL(hipMemcpyAsync(hs.data(), ds, hs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
