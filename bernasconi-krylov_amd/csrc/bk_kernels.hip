@@ -118,6 +118,13 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* scra
     }
 }
 
+// an LDS element by byte offset
+template <typename V>
+__device__ __forceinline__ V& lds_at(unsigned char* base, uint32_t byte_off)
+{
+    return *reinterpret_cast<V*>(base + byte_off);
+}
+
 // LDS swizzle: XOR the low SW bits of the element index with the next SW bits.
 template <int SW>
 __device__ __forceinline__ uint32_t swz(uint32_t u)
@@ -160,7 +167,6 @@ struct SweepArgs {
     void* dst2;              // its output (null: in place)
     double bc, bs;     // butterfly coefficients (F_TAN: bs = tan, bc unused)
     double bc2, bs2;   // F_POSTROT: coefficients of the rotations after the phase
-    double scale;      // overall factor applied with the phase (c^n in F_TAN form)
     int n;
     // HIGH: the active local bits [R, TB) map to two global segments,
     // [h, h+a1) then [h2, h2+(TB-R-a1)); the tile index fills the remaining
@@ -179,6 +185,23 @@ struct SweepArgs {
     const uint32_t* s0_arr;     // per state
     const uint32_t* e0_arr;     // per state
 };
+
+// entries of a phase table for n spins: E = 4m + c with E <= sum_{j<n} j^2
+__host__ __device__ constexpr int phase_tab_count(int n)
+{
+    return static_cast<int>(((static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6) >> 2) + 1);
+}
+
+// complex64 high-qubit passes that apply the phase copy the phase table into
+// LDS behind the 2^14-amplitude exchange buffer (at most 2605 entries, 20.8 KB
+// at n = 32): the per-amplitude table reads become LDS gathers instead of
+// scattered global loads through the vector memory pipeline the state
+// streams use.  Batched passes keep per-state tables in global memory.
+template <typename V, bool LOW, int FLAGS>
+constexpr bool lds_phase_table()
+{
+    return !LOW && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 && sizeof(V) == 8;
+}
 
 template <int TB_, int RB_, int R_>
 struct HighPlan {
@@ -218,8 +241,21 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         dp[1] = a.dst2 ? static_cast<V*>(a.dst2) : sp[1];
     }
     constexpr uint32_t ROWMASK = (1u << RR) - 1u;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    V* lds = reinterpret_cast<V*>(smem_raw);
+    // Static LDS: its offset is a compile-time constant that the DS
+    // instructions absorb, where a dynamic buffer's link-time base costs one
+    // more add per access.  Phase passes of complex64 high-qubit groups keep
+    // the phase table (up to n = 32) behind the exchange buffer.
+    // A one-phase plan exchanges nothing: it only needs the reduction
+    // scratch (NW x 3 doubles), which leaves its occupancy to the registers.
+    constexpr bool LDS_TAB = lds_phase_table<V, P.low, FLAGS>();
+    constexpr int XCH = NP > 1 ? 1 << TB : 0;
+    constexpr int SCRATCH = (3 * NW * 8 + static_cast<int>(sizeof(V)) - 1) / static_cast<int>(sizeof(V));
+    constexpr int LDS_ELEMS = XCH + (LDS_TAB ? phase_tab_count(32) : 0) > SCRATCH
+                                  ? XCH + (LDS_TAB ? phase_tab_count(32) : 0)
+                                  : SCRATCH;
+    __shared__ __align__(16) V lds_buf[LDS_ELEMS];
+    unsigned char* smem_raw = reinterpret_cast<unsigned char*>(lds_buf);
+    V* lds = lds_buf;
     const uint32_t t = threadIdx.x;
 
     uint32_t tu[NP];
@@ -242,11 +278,13 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     };
     const uint64_t tg0 = gl(tu[0]);
     const uint64_t tgL = gl(tu[NP - 1]);
-    // Two-state HIGH sweeps address register j as the sum of the (uniform)
-    // strides of its register bits, rebuilt per tile on the scalar unit: kept
-    // across the loop, the 2 x NR 64-bit offsets would not fit beside 2 x NR
-    // amplitudes in the register file.
-    constexpr bool STRIDED = !LOW && NS == 2;
+    // HIGH sweeps address register j as the sum of the (uniform) strides of
+    // its register bits, rebuilt per tile: kept across the loop, the NR (two
+    // states: 2 x NR) 64-bit offsets overflow the scalar registers, and their
+    // spill reloads cost more vector instructions than the additions.
+    // Batched sweeps keep the per-register offsets: their per-tile state
+    // parameters leave no vector registers for the rebuilt addresses.
+    constexpr bool STRIDED = !LOW && !BATCH;
     uint64_t lstr[RB], sstr[RB];
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
@@ -261,6 +299,15 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
         return o;
     };
     const R bc = static_cast<R>(a.bc), bs = static_cast<R>(a.bs);
+    [[maybe_unused]] const V* ltab = nullptr;
+    if constexpr (LDS_TAB) {
+        V* tl = lds + XCH;
+        const V* gt = static_cast<const V*>(a.tab);
+        const int cnt = phase_tab_count(a.n);
+        for (int i = static_cast<int>(t); i < cnt; i += 1 << TT) tl[i] = gt[i];
+        __syncthreads();
+        ltab = tl;
+    }
     double acc[3] = {0.0, 0.0, 0.0};
     double gacc[1] = {0.0};
     double gacc2[1] = {0.0};
@@ -378,31 +425,23 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             if (p > 0) {
-                if constexpr (NS == 2 || POST) {
-                    // swz is linear over XOR and the thread and register bits are
-                    // disjoint: swz(tu | pat) = swz(tu) ^ swz(pat).  The thread
-                    // part is made opaque per tile so the 2 x NR addresses of
-                    // each phase are one XOR each instead of registers held
-                    // across the loop.
-                    uint32_t wb = swz<SW>(tu[p - 1]), rb = swz<SW>(tu[p]);
-                    asm volatile("" : "+v"(wb), "+v"(rb));
+                // swz is linear over XOR and the thread and register bits are
+                // disjoint: swz(tu | pat) = swz(tu) ^ swz(pat), and so are the
+                // byte offsets (sizeof(V) is a power of two).  The thread part
+                // is made opaque per tile, so each access is one XOR with a
+                // constant instead of an address held across the loop.
+                uint32_t wb = swz<SW>(tu[p - 1]) * sizeof(V), rb = swz<SW>(tu[p]) * sizeof(V);
+                asm volatile("" : "+v"(wb), "+v"(rb));
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        __syncthreads();
-#pragma unroll
-                        for (int j = 0; j < NR; ++j) lds[wb ^ swz<SW>(reg_pattern(P.ph[p - 1], RB, j))] = x[s][j];
-                        __syncthreads();
-#pragma unroll
-                        for (int j = 0; j < NR; ++j) x[s][j] = lds[rb ^ swz<SW>(reg_pattern(P.ph[p], RB, j))];
-                    }
-                } else {
+                for (int s = 0; s < NS; ++s) {
                     __syncthreads();
 #pragma unroll
                     for (int j = 0; j < NR; ++j)
-                        lds[swz<SW>(tu[p - 1] | reg_pattern(P.ph[p - 1], RB, j))] = x[0][j];
+                        lds_at<V>(smem_raw, wb ^ (swz<SW>(reg_pattern(P.ph[p - 1], RB, j)) * sizeof(V))) = x[s][j];
                     __syncthreads();
 #pragma unroll
-                    for (int j = 0; j < NR; ++j) x[0][j] = lds[swz<SW>(tu[p] | reg_pattern(P.ph[p], RB, j))];
+                    for (int j = 0; j < NR; ++j)
+                        x[s][j] = lds_at<V>(smem_raw, rb ^ (swz<SW>(reg_pattern(P.ph[p], RB, j)) * sizeof(V)));
                 }
             }
 #pragma unroll
@@ -432,7 +471,6 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                 e0v = a.e0_arr[inst];
                 s0v = a.s0_arr[inst];
             }
-            const R sc = static_cast<R>(a.scale);
             R tacc[3] = {R(0), R(0), R(0)};
             // d_H weight of register j: s = T | off_j with T = base + tgL (thread and
             // tile bits) and off_j on the register bits M, disjoint, so
@@ -457,8 +495,9 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                 if constexpr ((FLAGS & F_ECACHE) != 0) E = (ec[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
                 else E = labs_energy_u32(static_cast<uint32_t>(base + tgL + gl(reg_pattern(P.ph[NP - 1], RB, j))), a.n);
                 if constexpr ((FLAGS & F_PHASE) != 0) {
-                    const V ph = tab[E >> 2];  // E = 4m + (n/2 mod 4): one entry per m
-                    const R pr = ph.x * sc, pi = ph.y * sc;
+                    // E = 4m + c_n: one entry per m; the table carries the step's scale
+                    const V ph = LDS_TAB ? ltab[E >> 2] : tab[E >> 2];
+                    const R pr = ph.x, pi = ph.y;
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         const R xr = x[s][j].x, xi = x[s][j].y;
@@ -489,16 +528,18 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
             for (int p = NP - 1; p >= 0; --p) {
                 if (p < NP - 1) {
-                    uint32_t wb = swz<SW>(tu[p + 1]), rb = swz<SW>(tu[p]);
+                    uint32_t wb = swz<SW>(tu[p + 1]) * sizeof(V), rb = swz<SW>(tu[p]) * sizeof(V);
                     asm volatile("" : "+v"(wb), "+v"(rb));
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         __syncthreads();
 #pragma unroll
-                        for (int j = 0; j < NR; ++j) lds[wb ^ swz<SW>(reg_pattern(P.ph[p + 1], RB, j))] = x[s][j];
+                        for (int j = 0; j < NR; ++j)
+                            lds_at<V>(smem_raw, wb ^ (swz<SW>(reg_pattern(P.ph[p + 1], RB, j)) * sizeof(V))) = x[s][j];
                         __syncthreads();
 #pragma unroll
-                        for (int j = 0; j < NR; ++j) x[s][j] = lds[rb ^ swz<SW>(reg_pattern(P.ph[p], RB, j))];
+                        for (int j = 0; j < NR; ++j)
+                            x[s][j] = lds_at<V>(smem_raw, rb ^ (swz<SW>(reg_pattern(P.ph[p], RB, j)) * sizeof(V)));
                     }
                 }
 #pragma unroll
@@ -997,23 +1038,12 @@ double exact_scale(const MixCoef& mc, int n)
     return std::pow(c * c + b * b, -0.5 * n);
 }
 
-template <typename K>
-hipError_t set_lds(K kernel, size_t bytes)
-{
-    if (bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
-}
-
 template <typename V, int FLAGS>
 hipError_t launch_low(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
     auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FLAGS, C::SW>;
-    const size_t lds = sizeof(V) << C::TB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, psi, a);
     return hipGetLastError();
 }
 
@@ -1037,10 +1067,7 @@ hipError_t launch_high_r(V* psi, const SweepArgs& a, int grid, hipStream_t st)
     using C = Cfg<V>;
     using PH = HighPlan<C::HTB, C::HRB, R>;
     auto k = k_sweep<V, PH, (TAN ? F_TAN : 0) | (PF ? F_PREFETCH : 0), C::SW>;
-    const size_t lds = sizeof(V) << C::HTB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, psi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
 }
 
@@ -1128,7 +1155,9 @@ hipError_t step_impl(V* psi, V* alt, int* in_alt, const StepSpec& sp, const Swee
     a.partials = partials;
     a.bc = mc.bc;
     a.bs = mc.bs;
-    a.scale = mc.scale;
+    // the tiled sweeps take the step's overall factor (c^n in tan form) from the
+    // error-feedback table
+    if (mc.scale != 1.0) return hipErrorInvalidValue;
     a.n = n;
     a.s0 = sp.s0;
     a.e0 = sp.e0;
@@ -1190,10 +1219,7 @@ hipError_t launch_high_rev_r(V* phi, const SweepArgs& a, int grid, hipStream_t s
     using C = Cfg<V>;
     using PH = HighPlan<C::HTB, C::HRB, R>;
     auto k = k_sweep<V, PH, F_GRAD, C::SW, 2>;
-    const size_t lds = sizeof(V) << C::HTB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, phi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, phi, a);
     return hipGetLastError();
 }
 
@@ -1213,10 +1239,7 @@ hipError_t launch_low_rev(V* phi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
     auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW, 2>;
-    const size_t lds = sizeof(V) << C::TB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, phi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, phi, a);
     return hipGetLastError();
 }
 
@@ -1251,7 +1274,6 @@ hipError_t reverse_step_impl(V* phi, V* chi, const StepSpec& sp, const SweepTuni
     a.tab = sp.phase_tab;
     a.bc = bc;
     a.bs = bs;
-    a.scale = 1.0;
     a.n = n;
     a.ecache = sp.ecache;
     a.psi2 = chi;
@@ -1306,10 +1328,7 @@ hipError_t launch_high_batch_r(V* psi, const SweepArgs& a, int grid, hipStream_t
     using C = Cfg<V>;
     using PH = HighPlan<C::HTB, C::HRB, R>;
     auto k = k_sweep<V, PH, F_BATCH | (TAN ? F_TAN : 0), C::SW>;
-    const size_t lds = sizeof(V) << C::HTB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, psi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
 }
 
@@ -1329,10 +1348,7 @@ hipError_t launch_low_batch(V* psi, const SweepArgs& a, int grid, hipStream_t st
 {
     using C = Cfg<V>;
     auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW>;
-    const size_t lds = sizeof(V) << C::TB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, psi, a);
     return hipGetLastError();
 }
 
@@ -1346,7 +1362,6 @@ hipError_t batch_step_impl(const BatchGroup& g, const SweepTuning& tun, double* 
     a.tab = g.tab;
     a.tab_stride = g.tab_stride;
     a.ecache = g.ecache;
-    a.scale = 1.0;
     a.n = n;
     a.bc_arr = g.bc;
     a.bs_arr = g.bs;
@@ -1398,10 +1413,7 @@ hipError_t launch_low_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
     auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW>;
-    const size_t lds = sizeof(V) << C::TB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), lds, st, psi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, psi, a);
     return hipGetLastError();
 }
 
@@ -1411,10 +1423,7 @@ hipError_t launch_high_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
     using C = Cfg<V>;
     using PH = HighPlan<C::HTB, C::HRB, R>;
     auto k = k_sweep<V, PH, FL, C::SW>;
-    const size_t lds = sizeof(V) << C::HTB;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, psi, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
 }
 
@@ -1522,7 +1531,6 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
     SweepArgs a{};
     a.tab = rs.phase_tab;
     a.partials = partials;
-    a.scale = 1.0;
     a.n = n;
     a.s0 = rs.s0;
     a.e0 = rs.e0;
@@ -1609,10 +1617,7 @@ hipError_t launch_high_rev_ppass(V* phi, int rbits, const SweepArgs& a, int grid
         if (rbits != R) return launch_high_rev_ppass<V, R + 1>(phi, rbits, a, grid, st);
         using PH = HighPlan<C::HTB, C::HRB, R>;
         auto k = k_sweep<V, PH, kRevPass, C::SW, 2>;
-        const size_t lds = sizeof(V) << C::HTB;
-        hipError_t e = set_lds(k, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), lds, st, phi, a);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, phi, a);
         return hipGetLastError();
     }
 }
@@ -1653,7 +1658,6 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
     const HighSweep B = grp.back();
     SweepArgs a{};
     a.tab = rs.phase_tab;
-    a.scale = 1.0;
     a.n = n;
     a.psi2 = chi;
     const int gL = clamp_grid(dim >> C::TB, tun.grid), gH = clamp_grid(dim >> C::HTB, tun.grid);
@@ -1734,7 +1738,6 @@ hipError_t batch_pass_impl(const BatchGroup& g, const BatchPass& bp, const Sweep
     SweepArgs a{};
     a.tab = g.tab;
     a.tab_stride = g.tab_stride;
-    a.scale = 1.0;
     a.n = n;
     a.bc_arr = g.bc;
     a.bs_arr = g.bs;

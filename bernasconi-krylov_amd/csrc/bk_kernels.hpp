@@ -39,7 +39,9 @@ struct StepSpec {
     // quotient target / rounded, in FP64; sd is the step's norm-exact mixer
     // scale.  The rounding residual of every entry is carried into the next
     // step, so the product of the applied factors follows the exact product to
-    // one rounding instead of drifting by one rounding per step.
+    // one rounding instead of drifting by one rounding per step.  Tiled steps
+    // (n >= the LOW tile bits) in tan form need it: the sweeps take the
+    // step's c^n from the table (hipErrorInvalidValue without one).
     const double* fb_master = nullptr;  // double2 exp(-i dtp E_m/n) per entry
     double* fb_carry = nullptr;         // double2 per entry
     int fb_count = 0;
