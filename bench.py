@@ -42,6 +42,14 @@ def _cpu_baseline(n: int, seed: int, threads: int):
     }
 
 
+def _step_algorithmic_bytes(n: int, precision: str) -> float:
+    """2 x b x 2^n x ceil(n / q) bytes for one Trotter step: every amplitude read
+    and written once per sweep, q = 14 (complex64) or 13 (complex128) qubits per
+    sweep, the largest tile that fits in 160 KB of LDS."""
+    b, q = (8, 14) if precision == "c64" else (16, 13)
+    return 2.0 * b * float(1 << n) * -(-n // q)
+
+
 def _traffic(kernel: str, n: int, precision: str, energy: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel calibration), for
@@ -178,6 +186,9 @@ def main() -> int:
                     help="skip the 512-trap batched timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 8 GiB device copy (HBM ceiling / PMC calibration)")
+    ap.add_argument("--cache-sweep", action="store_true",
+                    help="only run the plain copy over working sets from 16 MB to 8 GiB "
+                         "(how much faster a cache-resident working set streams than HBM)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -187,6 +198,14 @@ def main() -> int:
         return 2
 
     bk = importlib.import_module("bernasconi-krylov_amd")
+    if args.cache_sweep:
+        eng = bk.Engine(0)
+        rows = {}
+        for mb in (8, 16, 32, 64, 128, 256, 1024, 8192):  # bytes copied each way; working set is twice that
+            rows[f"{2 * mb} MB working set"] = eng.copy_probe(mb << 20, 50 if mb <= 256 else 5)
+        print(json.dumps({"copy_gbs_read_plus_write": rows}))
+        eng.close()
+        return 0
     if args.copy_probe:
         eng = bk.Engine(0)
         print(json.dumps({"copy_probe_bytes": 1 << 33, "gbs_read_plus_write": eng.copy_probe(1 << 33, 5)}))
@@ -268,6 +287,13 @@ def main() -> int:
             "p_less_final": float(res["p_less"][-1]), "norm_final": float(res["norm"][-1]),
             "step_hbm_bytes": step_bytes,
             "step_hbm_frac": step_bytes / (step_dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            # the step's algorithmic bytes as a sweep-per-qubit-group schedule
+            # moves them: 3 sweeps x (8 B read + 8 B written) per amplitude for
+            # complex64 at N <= 32 (a 2^14-amplitude LDS tile rotates at most 14
+            # qubits), against the measured time of one step
+            "step_algorithmic_bytes": _step_algorithmic_bytes(n, args.precision),
+            "step_algorithmic_frac": _step_algorithmic_bytes(n, args.precision) / (ms_per_step * 1e-3) / 1e9
+                                     / PEAK_HBM_GBS,
             "kernels": kernels,
         },
         "roofline": {

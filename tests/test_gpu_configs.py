@@ -5,6 +5,8 @@
   error shrinking as dt^2 (P_< and <d_H> are diagonal, where Lie and Strang agree);
 * N=24, d_min >= 6 trap: 1000 Strang steps in complex64 against complex128
   (P_<, <d_H> every 10 steps; state overlap every 100);
+* N=32 (34.4 GB complex64, 68.7 GB complex128): 20 Strang steps in complex64
+  against complex128 and the norm, on the largest state the engine takes;
 * N=26: P_<(T) of Strang dt=0.02 against the FP64 matrix-free Krylov
   propagator (m = 30).  At N=26 the spectrum of H = E/N - g sum X spans ~220,
   and restarting every 0.5 leaves the m=30 Krylov step unconverged (measured:
@@ -63,6 +65,27 @@ def test_n24_c64_against_c128(bk):
     finally:
         e64.close()
         e128.close()
+
+
+@pytest.mark.slow
+def test_n32_c64_against_c128(bk):
+    n = 32
+    _, s0, e0 = bk.make_trap(n, 1)
+    out = {}
+    for prec in (bk.BK_C64, bk.BK_C128):
+        eng = bk.Engine(0)
+        try:
+            out[prec] = eng.evolve(n, s0, 0.35, 0.02, 20, order=bk.BK_STRANG, precision=prec, e0=e0,
+                                   measure_every=5)
+        finally:
+            eng.close()
+    a, b = out[bk.BK_C64], out[bk.BK_C128]
+    print(f"\n[N=32] P_< c64 {a['p_less'][-1]:.10e} c128 {b['p_less'][-1]:.10e}; "
+          f"norm c64 {a['norm'][-1]:.12f}")
+    assert np.abs(a["p_less"] - b["p_less"]).max() <= 1e-6
+    assert np.abs(a["dh"] - b["dh"]).max() <= 1e-5
+    assert np.abs(b["norm"] - 1.0).max() <= 1e-12
+    assert np.abs(a["norm"] - 1.0).max() <= 1e-5
 
 
 @pytest.mark.slow
