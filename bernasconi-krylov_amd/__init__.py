@@ -117,6 +117,13 @@ class _SamplingResultC(C.Structure):
                 ("samples", C.POINTER(C.c_int64)), ("samples_len", C.c_int64)]
 
 
+class _TrapRecordC(C.Structure):
+    _fields_ = [("n", C.c_int), ("g", C.c_double), ("trap_rank", C.c_int64), ("trap_bits", C.c_uint64),
+                ("canonical_bits", C.c_uint64), ("trap_energy", C.c_int64), ("d_min", C.c_int),
+                ("quantum_time_sec", C.c_double), ("window_t_start", C.c_double), ("window_points", C.c_int),
+                ("window_dt", C.c_double), ("t_max", C.c_double), ("result", C.POINTER(_SamplingResultC))]
+
+
 class KernelTiming(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int64), ("total_ms", C.c_double),
                 ("bytes_per_launch", C.c_double)]
@@ -167,6 +174,9 @@ _SIGS = {
     "bk_sweep_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     "bk_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "bk_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "bk_export_results_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
+    "bk_export_time_series_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
+    "bk_export_structured_json": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int, C.c_int]),
     "bk_timing_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTiming), C.c_int,
                                  C.POINTER(C.c_int)]),
 }

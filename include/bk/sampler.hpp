@@ -122,6 +122,44 @@ private:
     bk_ctx* ctx_ = nullptr;
 };
 
+// A library-owned bk_sampling_result as the reference's SamplingResult.
+inline SamplingResult to_sampling_result(const bk_sampling_result& r)
+{
+    SamplingResult out;
+    out.unique_basis.reserve(static_cast<size_t>(r.unique_basis_len));
+    for (int64_t i = 0; i < r.unique_basis_len; ++i) out.unique_basis.emplace_back(r.unique_basis[i]);
+    out.total_shots_collected = r.total_shots_collected;
+    out.improved_shots = r.improved_shots;
+    out.success_rate = r.success_rate;
+    out.per_eval_rate = r.per_eval_rate;
+    out.total_evals = r.total_evals;
+    out.mean_hamming_flips = r.mean_hamming_flips;
+    for (int64_t i = 0; i < r.seed_stats_len; ++i) {
+        const bk_seed_stats& s = r.seed_stats[i];
+        SeedSamplingStats o;
+        o.seed = s.seed;
+        o.total_shots_collected = s.total_shots_collected;
+        o.improved_shots = s.improved_shots;
+        o.success_rate = s.success_rate;
+        o.per_eval_rate = s.per_eval_rate;
+        o.mean_hamming_flips = s.mean_hamming_flips;
+        o.best_energy = s.best_energy;
+        out.seed_stats.push_back(o);
+    }
+    out.time_points.assign(r.time_points, r.time_points + r.history_len);
+    out.p_less_history.assign(r.p_less_history, r.p_less_history + r.history_len);
+    out.dh_history.assign(r.dh_history, r.dh_history + r.history_len);
+    out.window_avg_dh = r.window_avg_dh;
+    out.initial_rise_rate_avg = r.initial_rise_rate_avg;
+    out.initial_rise_rate_max = r.initial_rise_rate_max;
+    out.d_min = r.d_min;
+    out.start_energy = r.start_energy;
+    out.p_blind = r.p_blind;
+    out.peak_p_less = r.peak_p_less;
+    out.peak_time = r.peak_time;
+    return out;
+}
+
 // The GPU backend behind create_sampler (the role of CuStateVecSampler).
 class HipSampler : public ISampler {
 public:
@@ -151,39 +189,7 @@ public:
         bk_sampling_result* r = nullptr;
         check(bk_sampler_run(ctx_.get(), &cfg, start_state.bits, &r), "bk_sampler_run");
         std::unique_ptr<bk_sampling_result, void (*)(bk_sampling_result*)> guard(r, bk_result_free);
-        SamplingResult out;
-        out.unique_basis.reserve(static_cast<size_t>(r->unique_basis_len));
-        for (int64_t i = 0; i < r->unique_basis_len; ++i) out.unique_basis.emplace_back(r->unique_basis[i]);
-        out.total_shots_collected = r->total_shots_collected;
-        out.improved_shots = r->improved_shots;
-        out.success_rate = r->success_rate;
-        out.per_eval_rate = r->per_eval_rate;
-        out.total_evals = r->total_evals;
-        out.mean_hamming_flips = r->mean_hamming_flips;
-        for (int64_t i = 0; i < r->seed_stats_len; ++i) {
-            const bk_seed_stats& s = r->seed_stats[i];
-            SeedSamplingStats o;
-            o.seed = s.seed;
-            o.total_shots_collected = s.total_shots_collected;
-            o.improved_shots = s.improved_shots;
-            o.success_rate = s.success_rate;
-            o.per_eval_rate = s.per_eval_rate;
-            o.mean_hamming_flips = s.mean_hamming_flips;
-            o.best_energy = s.best_energy;
-            out.seed_stats.push_back(o);
-        }
-        out.time_points.assign(r->time_points, r->time_points + r->history_len);
-        out.p_less_history.assign(r->p_less_history, r->p_less_history + r->history_len);
-        out.dh_history.assign(r->dh_history, r->dh_history + r->history_len);
-        out.window_avg_dh = r->window_avg_dh;
-        out.initial_rise_rate_avg = r->initial_rise_rate_avg;
-        out.initial_rise_rate_max = r->initial_rise_rate_max;
-        out.d_min = r->d_min;
-        out.start_energy = r->start_energy;
-        out.p_blind = r->p_blind;
-        out.peak_p_less = r->peak_p_less;
-        out.peak_time = r->peak_time;
-        return out;
+        return to_sampling_result(*r);
     }
 
     bk_ctx* context() const { return ctx_.get(); }

@@ -325,6 +325,38 @@ int bk_timing_enable(bk_ctx* ctx, int enable);
 int bk_copy_probe(bk_ctx* ctx, uint64_t bytes, int reps, double* gbs);
 int bk_timing_read(bk_ctx* ctx, bk_kernel_timing* out, int cap, int* count);
 
+/* ---- the benchmark connector's output files (connector.cuh:839-1155) ----
+ * One record per (trap, g): what BenchmarkConnector::run keeps of a trap and
+ * its sampler result (SingleTrapBenchmark, connector.cuh:26-117, quantum part).
+ * Host-only; the files are byte-for-byte the reference writer's for the same
+ * values (include/bk/wire.hpp holds the C++ writers these call). */
+typedef struct bk_trap_record {
+    int n;
+    double g;
+    int64_t trap_rank;                 /* 1-based rank of the trap */
+    uint64_t trap_bits;                /* trap state s0 */
+    uint64_t canonical_bits;           /* canonical representative of its symmetry orbit */
+    int64_t trap_energy;               /* E0 */
+    int d_min;                         /* 0: taken from the result */
+    double quantum_time_sec;           /* wall time of the sampler call */
+    /* window of window_avg_p_less (connector.cuh:816-830): [t_start, t_start +
+     * (window_points - 1) window_dt], or [t_start, t_max] when window_points < 2
+     * or window_dt <= 0 */
+    double window_t_start;
+    int window_points;
+    double window_dt;
+    double t_max;
+    const bk_sampling_result* result;  /* the sampler's result (bk_sampler_run); null: zeros */
+} bk_trap_record;
+
+/* export_results_csv (connector.cuh:839-904): header + one row per record */
+int bk_export_results_csv(const char* path, const bk_trap_record* recs, int count);
+/* export_time_series_csv (connector.cuh:906-933): one row per recorded time */
+int bk_export_time_series_csv(const char* path, const bk_trap_record* recs, int count);
+/* export_structured_json (connector.cuh:935-1155): records grouped by
+ * (n, trap_rank); n_override = 0 takes N from the first record */
+int bk_export_structured_json(const char* path, const bk_trap_record* recs, int count, int n_override);
+
 #ifdef __cplusplus
 }
 #endif
