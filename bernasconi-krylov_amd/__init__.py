@@ -174,6 +174,12 @@ _SIGS = {
     "bk_sweep_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     "bk_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "bk_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "bk_subspace_lanczos": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_uint64), C.c_int64,
+                                      C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bk_subspace_dense": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_uint64), C.c_int64,
+                                    C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bk_fullspace_dense": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "bk_export_results_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
     "bk_export_time_series_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
     "bk_export_structured_json": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int, C.c_int]),
@@ -510,6 +516,42 @@ class Engine:
         vec = np.zeros((k_pairs, 1 << n)) if vectors else None
         k = _check(lib().bk_lanczos(self._h, n, g, 1 if normalize_by_n else 0, max_krylov_dim,
                                     tol, k_pairs, _dp(ev), _dp(vec) if vectors else None))
+        return ev[:k], (vec[:k] if vectors else None)
+
+    def _basis(self, basis) -> np.ndarray:
+        return np.ascontiguousarray(np.asarray(basis, dtype=np.uint64))
+
+    def subspace_lanczos(self, n: int, g: float, basis, max_krylov_dim: int = 100, tol: float = 1e-12,
+                         k_pairs: int = 1, normalize_by_n: bool = True, vectors: bool = True):
+        """Diagonalizer::solve(basis, g, n, cfg) with DiagMethod::Lanczos
+        (diagonalizer.cuh:299-372); eigenvectors in the order of basis."""
+        b = self._basis(basis)
+        ev = np.zeros(k_pairs)
+        vec = np.zeros((k_pairs, b.shape[0])) if vectors else None
+        k = _check(lib().bk_subspace_lanczos(self._h, n, g, 1 if normalize_by_n else 0,
+                                             b.ctypes.data_as(C.POINTER(C.c_uint64)), b.shape[0],
+                                             max_krylov_dim, tol, k_pairs, _dp(ev),
+                                             _dp(vec) if vectors else None))
+        return ev[:k], (vec[:k] if vectors else None)
+
+    def subspace_dense(self, n: int, g: float, basis, k_pairs: int = 1, normalize_by_n: bool = True,
+                       vectors: bool = True):
+        """DiagMethod::ExactDense on a basis (diagonalizer.cuh:188-245), rocSOLVER on the device."""
+        b = self._basis(basis)
+        ev = np.zeros(k_pairs)
+        vec = np.zeros((k_pairs, b.shape[0])) if vectors else None
+        k = _check(lib().bk_subspace_dense(self._h, n, g, 1 if normalize_by_n else 0,
+                                           b.ctypes.data_as(C.POINTER(C.c_uint64)), b.shape[0], k_pairs,
+                                           _dp(ev), _dp(vec) if vectors else None))
+        return ev[:k], (vec[:k] if vectors else None)
+
+    def fullspace_dense(self, n: int, g: float, k_pairs: int = 1, normalize_by_n: bool = True,
+                        vectors: bool = True):
+        """DiagMethod::ExactDense on the full space (diagonalizer.cuh:250-294), n <= 14."""
+        ev = np.zeros(k_pairs)
+        vec = np.zeros((k_pairs, 1 << n)) if vectors else None
+        k = _check(lib().bk_fullspace_dense(self._h, n, g, 1 if normalize_by_n else 0, k_pairs, _dp(ev),
+                                            _dp(vec) if vectors else None))
         return ev[:k], (vec[:k] if vectors else None)
 
     def krylov_propagate(self, g: float, tau: float, tau_restart: float = 0.5, m: int = 30,

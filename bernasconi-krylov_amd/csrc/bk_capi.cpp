@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <iomanip>
 #include <iostream>
 #include <map>
@@ -205,6 +206,85 @@ int bkc::ensure_ecache(bk_ctx* c)
     }
     return BK_OK;
 }
+
+// Plain Lanczos over a device matvec on vectors of length dim
+// (run_gpu_lanczos_internal, diagonalizer.cuh:420-554): v0 from mt19937_64(42),
+// uniform(-0.5, 0.5); no reorthogonalisation; breakdown at |beta| < tol; the
+// k_pairs lowest Ritz values (and, with evecs, the Ritz vectors, host, pair
+// after pair).  Returns the number of pairs.
+int bkc::lanczos_core(bk_ctx* c, int64_t dim, const std::function<hipError_t(const double*, double*)>& matvec,
+                      int max_krylov_dim, double tol, int k_pairs, double* evals, double* evecs)
+{
+    int rc = BK_OK;
+    const int m = static_cast<int>(std::min<int64_t>(dim, max_krylov_dim));
+    ScopedBuf V, w;
+    HIPCALL(V.ensure(sizeof(double) * static_cast<size_t>(dim) * m));
+    HIPCALL(w.ensure(sizeof(double) * static_cast<size_t>(dim)));
+    HIPCALL(c->blas_partials.ensure(sizeof(double) * bk::blas_partials_doubles()));
+    HIPCALL(c->blas_out.ensure(sizeof(double) * 4));
+    double* dV = V.as<double>();
+    double* dw = w.as<double>();
+    auto dot = [&](const double* x, const double* y, double* r) -> int {
+        HIPCALL(bk::launch_dot(dim, 0, x, y, c->blas_out.as<double>(), c->blas_partials.as<double>(), c->stream));
+        double h[2];
+        HIPCALL(hipMemcpyAsync(h, c->blas_out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
+        *r = h[0];
+        return BK_OK;
+    };
+    // v0: mt19937_64(42), uniform(-0.5, 0.5)  (diagonalizer.cuh:441-446)
+    {
+        std::mt19937_64 rng(42);
+        std::uniform_real_distribution<double> dist(-0.5, 0.5);
+        std::vector<double> v0(static_cast<size_t>(dim));
+        for (auto& x : v0) x = dist(rng);
+        HIPCALL(hipMemcpy(dV, v0.data(), v0.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    double nrm2 = 0.0;
+    if ((rc = dot(dV, dV, &nrm2))) return rc;
+    HIPCALL(bk::launch_scal(dim, 1.0 / std::sqrt(nrm2), dV, c->stream));
+    std::vector<double> alpha(m, 0.0), beta(m, 0.0);
+    int m_act = m;
+    for (int j = 0; j < m; ++j) {
+        double* vj = dV + static_cast<size_t>(j) * dim;
+        HIPCALL(matvec(vj, dw));
+        if (j > 0) HIPCALL(bk::launch_axpy(dim, 0, -beta[j - 1], 0.0, vj - dim, dw, c->stream));
+        if ((rc = dot(vj, dw, &alpha[j]))) return rc;
+        HIPCALL(bk::launch_axpy(dim, 0, -alpha[j], 0.0, vj, dw, c->stream));
+        if (j < m - 1) {
+            double b2 = 0.0;
+            if ((rc = dot(dw, dw, &b2))) return rc;
+            beta[j] = std::sqrt(b2);
+            if (std::fabs(beta[j]) < tol) {
+                m_act = j + 1;
+                break;
+            }
+            double* vn = vj + dim;
+            HIPCALL(hipMemcpyAsync(vn, dw, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->stream));
+            HIPCALL(bk::launch_scal(dim, 1.0 / beta[j], vn, c->stream));
+        }
+    }
+    std::vector<double> d(alpha.begin(), alpha.begin() + m_act);
+    std::vector<double> e(beta.begin(), beta.begin() + std::max(0, m_act - 1));
+    std::vector<double> z;
+    if (!bk::tridiag_eigen(d, e, &z)) return fail(BK_E_INVALID, "tridiagonal eigensolver did not converge");
+    const int kp = std::min(k_pairs, m_act);
+    for (int k = 0; k < kp; ++k) evals[k] = d[k];
+    if (evecs) {
+        ScopedBuf coef;
+        HIPCALL(coef.ensure(sizeof(double) * m_act));
+        for (int k = 0; k < kp; ++k) {
+            HIPCALL(hipMemcpy(coef.p, z.data() + static_cast<size_t>(k) * m_act, sizeof(double) * m_act,
+                              hipMemcpyHostToDevice));
+            HIPCALL(bk::launch_combine(dim, 0, m_act, dV, coef.as<double>(), dw, c->stream));
+            HIPCALL(hipMemcpyAsync(evecs + static_cast<size_t>(k) * dim, dw, sizeof(double) * dim,
+                                   hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCALL(hipStreamSynchronize(c->stream));
+    }
+    return kp;
+}
+
 
 extern "C" {
 
@@ -789,77 +869,10 @@ int bk_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, int max_krylov_di
     if (rc) return rc;
     if (n < 1 || n > 30 || max_krylov_dim < 1 || k_pairs < 1 || !evals || !is_fin(g))
         return fail(BK_E_INVALID, "bk_lanczos: bad argument");
-    const int64_t dim = 1ll << n;
-    const int m = static_cast<int>(std::min<int64_t>(dim, max_krylov_dim));
-    DevBuf V, w;
-    HIPCALL(V.ensure(sizeof(double) * static_cast<size_t>(dim) * m));
-    HIPCALL(w.ensure(sizeof(double) * static_cast<size_t>(dim)));
-    HIPCALL(c->blas_partials.ensure(sizeof(double) * bk::blas_partials_doubles()));
-    HIPCALL(c->blas_out.ensure(sizeof(double) * 4));
-    double* dV = V.as<double>();
-    double* dw = w.as<double>();
-    auto dot = [&](const double* x, const double* y, double* r) -> int {
-        HIPCALL(bk::launch_dot(dim, 0, x, y, c->blas_out.as<double>(), c->blas_partials.as<double>(), c->stream));
-        double h[2];
-        HIPCALL(hipMemcpyAsync(h, c->blas_out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCALL(hipStreamSynchronize(c->stream));
-        *r = h[0];
-        return BK_OK;
-    };
-    // v0: mt19937_64(42), uniform(-0.5, 0.5)  (diagonalizer.cuh:441-446)
-    {
-        std::mt19937_64 rng(42);
-        std::uniform_real_distribution<double> dist(-0.5, 0.5);
-        std::vector<double> v0(static_cast<size_t>(dim));
-        for (auto& x : v0) x = dist(rng);
-        HIPCALL(hipMemcpy(dV, v0.data(), v0.size() * sizeof(double), hipMemcpyHostToDevice));
-    }
-    double nrm2 = 0.0;
-    if ((rc = dot(dV, dV, &nrm2))) return rc;
-    HIPCALL(bk::launch_scal(dim, 1.0 / std::sqrt(nrm2), dV, c->stream));
-    std::vector<double> alpha(m, 0.0), beta(m, 0.0);
-    int m_act = m;
-    for (int j = 0; j < m; ++j) {
-        double* vj = dV + static_cast<size_t>(j) * dim;
-        HIPCALL(bk::launch_hv(n, g, normalize_by_n, 0, vj, dw, c->stream));
-        if (j > 0) HIPCALL(bk::launch_axpy(dim, 0, -beta[j - 1], 0.0, vj - dim, dw, c->stream));
-        if ((rc = dot(vj, dw, &alpha[j]))) return rc;
-        HIPCALL(bk::launch_axpy(dim, 0, -alpha[j], 0.0, vj, dw, c->stream));
-        if (j < m - 1) {
-            double b2 = 0.0;
-            if ((rc = dot(dw, dw, &b2))) return rc;
-            beta[j] = std::sqrt(b2);
-            if (std::fabs(beta[j]) < tol) {
-                m_act = j + 1;
-                break;
-            }
-            double* vn = vj + dim;
-            HIPCALL(hipMemcpyAsync(vn, dw, sizeof(double) * dim, hipMemcpyDeviceToDevice, c->stream));
-            HIPCALL(bk::launch_scal(dim, 1.0 / beta[j], vn, c->stream));
-        }
-    }
-    std::vector<double> d(alpha.begin(), alpha.begin() + m_act);
-    std::vector<double> e(beta.begin(), beta.begin() + std::max(0, m_act - 1));
-    std::vector<double> z;
-    if (!bk::tridiag_eigen(d, e, &z)) return fail(BK_E_INVALID, "tridiagonal eigensolver did not converge");
-    const int kp = std::min(k_pairs, m_act);
-    for (int k = 0; k < kp; ++k) evals[k] = d[k];
-    if (evecs) {
-        DevBuf coef;
-        HIPCALL(coef.ensure(sizeof(double) * m_act));
-        for (int k = 0; k < kp; ++k) {
-            HIPCALL(hipMemcpy(coef.p, z.data() + static_cast<size_t>(k) * m_act, sizeof(double) * m_act,
-                              hipMemcpyHostToDevice));
-            HIPCALL(bk::launch_combine(dim, 0, m_act, dV, coef.as<double>(), dw, c->stream));
-            HIPCALL(hipMemcpyAsync(evecs + static_cast<size_t>(k) * dim, dw, sizeof(double) * dim,
-                                   hipMemcpyDeviceToHost, c->stream));
-        }
-        HIPCALL(hipStreamSynchronize(c->stream));
-        coef.release();
-    }
-    V.release();
-    w.release();
-    return kp;
+    return lanczos_core(
+        c, 1ll << n,
+        [&](const double* x, double* y) { return bk::launch_hv(n, g, normalize_by_n, 0, x, y, c->stream); },
+        max_krylov_dim, tol, k_pairs, evals, evecs);
 }
 
 int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, double tau_restart, int m)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
 #include <utility>
@@ -55,6 +56,15 @@ struct DevBuf {
     {
         return static_cast<T*>(p);
     }
+};
+
+// A DevBuf freed when it goes out of scope (device memory local to one call,
+// released on every return path).
+struct ScopedBuf : DevBuf {
+    ScopedBuf() = default;
+    ScopedBuf(const ScopedBuf&) = delete;
+    ScopedBuf& operator=(const ScopedBuf&) = delete;
+    ~ScopedBuf() { release(); }
 };
 
 // HIP-event timer: events are drawn from a pool, recorded around each launch,
@@ -184,6 +194,10 @@ bool is_fin(double x);
 std::vector<double> phase_master(int n, double dtp);
 // the uint16 E(s) cache of the held state's n (LOW-sweep layout of its precision)
 int ensure_ecache(bk_ctx* c);
+// plain Lanczos (diagonalizer.cuh:420-554) over a device matvec on vectors of
+// length dim; returns the number of pairs written to evals (and evecs, host)
+int lanczos_core(bk_ctx* c, int64_t dim, const std::function<hipError_t(const double*, double*)>& matvec,
+                 int max_krylov_dim, double tol, int k_pairs, double* evals, double* evecs);
 
 
 }  // namespace bkc

@@ -82,9 +82,11 @@ hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepS
 // A phase-carrying pass rotates its tile's qubits for step k, applies P(k)
 // (and measures), then rotates them again for step k+1; the X_q of one step
 // commute, so only P has to sit between the steps.  With the high qubits in
-// one sweep group H (n <= 24 for complex64) a run of K steps takes K + 1 passes:
+// one sweep group H (complex64 n = 15..23, complex128 n = 13..21) a run of K
+// steps takes K + 1 passes:
 //   H(1), then for k = 1..K a phase pass on LOW (odd k) or H (even k);
-// with two groups A, B (n = 25..32) it takes 2K + 1 passes:
+// with two groups A, B (complex64 n = 24..32, complex128 n = 22..32) it takes
+// 2K + 1 passes:
 //   LOW(1), then for k = 1..K: A(k) and a phase pass on B (odd k) / LOW (even k);
 // the last phase pass stops after P(K).  Other n (and the (c, s) butterfly)
 // run step by step (launch_step).  The error-feedback table and the per-step
@@ -245,6 +247,16 @@ hipError_t launch_scal(int64_t len, double a, double* x, hipStream_t stream);
 hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V,
                           const double* coef, double* y, hipStream_t stream);
 size_t blas_partials_doubles();
+
+// ---- subspace Hamiltonian over a sorted basis (diagonalizer.cuh:79-114, 188-245) ----
+// nbr[i * n + j] = index of basis[i] ^ 2^j in the sorted basis, or -1
+hipError_t launch_sub_neighbors(const uint64_t* basis, int64_t dim, int n, int32_t* nbr, hipStream_t stream);
+// y = diag .* x - g * (sum over neighbours)
+hipError_t launch_sub_hv(const double* diag, const int32_t* nbr, int64_t dim, int n, double g, const double* x,
+                         double* y, hipStream_t stream);
+// H (column-major, zero on entry) = diag on the diagonal, -g between neighbours
+hipError_t launch_sub_dense(const double* diag, const int32_t* nbr, int64_t dim, int n, double g, double* H,
+                            hipStream_t stream);
 
 // plain float4 copy of 'bytes' (multiple of 16): HBM ceiling probe
 hipError_t launch_copy_probe(const void* src, void* dst, uint64_t bytes, hipStream_t stream);

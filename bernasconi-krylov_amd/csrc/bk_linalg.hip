@@ -473,6 +473,93 @@ hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V, c
 }
 
 // ---------------------------------------------------------------------------
+// Subspace Hamiltonian (diagonalizer.cuh:79-114, 188-245).  The basis is held
+// sorted; nbr[i * n + j] is the index of basis[i] ^ 2^j in it, or -1.  The
+// table is built once per basis (one binary search per state and spin), so
+// every H.v of a Lanczos run is a gather instead of n searches per row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_sub_neighbors(const uint64_t* __restrict__ basis, int64_t dim, int n,
+                                                            int32_t* __restrict__ nbr)
+{
+    const int64_t total = dim * n;
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < total;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t i = k / n;
+        const int j = static_cast<int>(k - i * n);
+        const uint64_t target = basis[i] ^ (1ull << j);
+        int64_t lo = 0, hi = dim - 1;
+        int32_t found = -1;
+        while (lo <= hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            const uint64_t b = basis[mid];
+            if (b == target) {
+                found = static_cast<int32_t>(mid);
+                break;
+            }
+            if (b < target) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        nbr[k] = found;
+    }
+}
+
+// y_i = diag_i x_i - g * x_b for every neighbour b of i, subtracted in spin order
+__global__ __launch_bounds__(kThreads) void k_sub_hv(const double* __restrict__ diag, const int32_t* __restrict__ nbr,
+                                                     int64_t dim, int n, double g, const double* __restrict__ x,
+                                                     double* __restrict__ y)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double yi = diag[i] * x[i];
+        const int32_t* row = nbr + i * n;
+        for (int j = 0; j < n; ++j) {
+            const int32_t b = row[j];
+            if (b >= 0) yi -= g * x[b];
+        }
+        y[i] = yi;
+    }
+}
+
+// dense H, column-major dim x dim, from the table (H zero on entry)
+__global__ __launch_bounds__(kThreads) void k_sub_dense(const double* __restrict__ diag, const int32_t* __restrict__ nbr,
+                                                        int64_t dim, int n, double g, double* __restrict__ H)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        H[i + i * dim] = diag[i];
+        for (int j = 0; j < n; ++j) {
+            const int32_t b = nbr[i * n + j];
+            if (b >= 0) H[i + static_cast<int64_t>(b) * dim] = -g;
+        }
+    }
+}
+
+static int grid_for(int64_t work)
+{
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((work + kThreads - 1) / kThreads, 16384)));
+}
+
+hipError_t launch_sub_neighbors(const uint64_t* basis, int64_t dim, int n, int32_t* nbr, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sub_neighbors, dim3(grid_for(dim * n)), dim3(kThreads), 0, st, basis, dim, n, nbr);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub_hv(const double* diag, const int32_t* nbr, int64_t dim, int n, double g, const double* x,
+                         double* y, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sub_hv, dim3(grid_for(dim)), dim3(kThreads), 0, st, diag, nbr, dim, n, g, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub_dense(const double* diag, const int32_t* nbr, int64_t dim, int n, double g, double* H,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sub_dense, dim3(grid_for(dim)), dim3(kThreads), 0, st, diag, nbr, dim, n, g, H);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Plain copy (HBM ceiling probe and PMC calibration: known bytes per launch)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b,

@@ -1,15 +1,19 @@
-// bk/diagonalizer.hpp -- C++17 drop-in for the full-space part of the
-// reference's src/subspace/diagonalizer.cuh, implemented over libbk.so.
+// bk/diagonalizer.hpp -- C++17 drop-in for the reference's
+// src/subspace/diagonalizer.cuh, implemented over libbk.so.
 //
 //   DiagMethod / DiagConfig / Eigenpair / Eigensolution ... diagonalizer.cuh:122-144
 //   FullSpaceMatVec: the MatVecFunc of :400-406, callable as
 //       void(const double* d_x, double* d_y)  on device pointers (FP64 real),
 //     computing y = (E/N) x - g sum_j X_j x (fullspace_matvec_kernel_d, :60-77)
-//   Diagonalizer::solve(n, g, cfg) with DiagMethod::Lanczos ..... :173-183, 377-416
-//     (GPU Lanczos of :420-554: v0 from mt19937_64(42), uniform(-0.5, 0.5),
-//      no reorthogonalisation, breakdown at |beta| < tolerance, Ritz vectors)
-// Dense ED (DiagMethod::ExactDense) and the subspace solvers are not part of
-// this boundary; solve() throws std::runtime_error for them.
+//   Diagonalizer::solve(n, g, cfg)
+//     Lanczos ....... :377-416 (GPU Lanczos of :420-554: v0 from mt19937_64(42),
+//                     uniform(-0.5, 0.5), no reorthogonalisation, breakdown at
+//                     |beta| < tolerance, Ritz vectors)
+//     ExactDense .... :250-294 (n <= 14), rocSOLVER on the device instead of Eigen
+//   Diagonalizer::solve(basis, g, n, cfg): H restricted to the basis states
+//     Lanczos ....... :299-372 (the same Lanczos over the subspace H.v, :79-114)
+//     ExactDense .... :188-245 (rocSOLVER on the device, at most 32768 states)
+//   Eigenvectors of a subspace solve are in the order of the given basis.
 #pragma once
 
 #include <stdexcept>
@@ -70,27 +74,56 @@ public:
 
     Eigensolution solve(int n, double g, DiagConfig config = {}) const
     {
-        if (config.method != DiagMethod::Lanczos)
-            throw std::runtime_error("dense ED is not provided by the GPU engine");
-        if (n > 30) throw std::runtime_error("Full space dimension 2^N exceeds memory limit for N > 30.");
+        if (config.method == DiagMethod::ExactDense) {
+            if (n > 14) throw std::runtime_error("Full space dense ED exceeds feasible memory for N > 14. Use Lanczos.");
+        } else if (n > 30) {
+            throw std::runtime_error("Full space dimension 2^N exceeds memory limit for N > 30.");
+        }
         const size_t dim = size_t(1) << n;
         std::vector<double> ev(static_cast<size_t>(config.num_eigenpairs));
         std::vector<double> vec(static_cast<size_t>(config.num_eigenpairs) * dim);
-        const int k = bk_lanczos(ctx_.get(), n, g, config.normalize_by_n ? 1 : 0, config.max_krylov_dim,
-                                 config.tolerance, config.num_eigenpairs, ev.data(), vec.data());
-        bk::check(k, "bk_lanczos");
+        const int k = config.method == DiagMethod::ExactDense
+                          ? bk_fullspace_dense(ctx_.get(), n, g, config.normalize_by_n ? 1 : 0, config.num_eigenpairs,
+                                               ev.data(), vec.data())
+                          : bk_lanczos(ctx_.get(), n, g, config.normalize_by_n ? 1 : 0, config.max_krylov_dim,
+                                       config.tolerance, config.num_eigenpairs, ev.data(), vec.data());
+        bk::check(k, config.method == DiagMethod::ExactDense ? "bk_fullspace_dense" : "bk_lanczos");
+        return unpack(k, ev, vec, dim);
+    }
+
+    Eigensolution solve(const std::vector<SpinState>& basis, double g, int n, DiagConfig config = {}) const
+    {
+        const size_t dim = basis.size();
+        if (dim == 0) return {};
+        std::vector<uint64_t> bits(dim);
+        for (size_t i = 0; i < dim; ++i) bits[i] = basis[i].bits;
+        std::vector<double> ev(static_cast<size_t>(config.num_eigenpairs));
+        std::vector<double> vec(static_cast<size_t>(config.num_eigenpairs) * dim);
+        const int norm = config.normalize_by_n ? 1 : 0;
+        const int k = config.method == DiagMethod::ExactDense
+                          ? bk_subspace_dense(ctx_.get(), n, g, norm, bits.data(), static_cast<int64_t>(dim),
+                                              config.num_eigenpairs, ev.data(), vec.data())
+                          : bk_subspace_lanczos(ctx_.get(), n, g, norm, bits.data(), static_cast<int64_t>(dim),
+                                                config.max_krylov_dim, config.tolerance, config.num_eigenpairs,
+                                                ev.data(), vec.data());
+        bk::check(k, config.method == DiagMethod::ExactDense ? "bk_subspace_dense" : "bk_subspace_lanczos");
+        return unpack(k, ev, vec, dim);
+    }
+
+    bk_ctx* context() const { return ctx_.get(); }
+
+private:
+    static Eigensolution unpack(int k, const std::vector<double>& ev, const std::vector<double>& vec, size_t dim)
+    {
         Eigensolution sol;
         sol.pairs.resize(static_cast<size_t>(k));
         for (int i = 0; i < k; ++i) {
-            sol.pairs[i].energy = ev[i];
+            sol.pairs[i].energy = ev[static_cast<size_t>(i)];
             sol.pairs[i].state_vector.assign(vec.begin() + static_cast<std::ptrdiff_t>(i * dim),
                                              vec.begin() + static_cast<std::ptrdiff_t>((i + 1) * dim));
         }
         return sol;
     }
 
-    bk_ctx* context() const { return ctx_.get(); }
-
-private:
     bk::Context ctx_;
 };

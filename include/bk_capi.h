@@ -138,11 +138,10 @@ int bk_adjoint_grad(bk_ctx* ctx, const bk_adjoint_params* p, double w_pless, dou
  * its own field g with the common dt / order / precision; out[i] receives
  * P_<, <d_H>, norm of instance i after the last step (input order).  The
  * states live in their own device arena (the context's held state is not
- * touched).  Each instance runs the same kernels and the same arithmetic as a
- * step-by-step bk_evolve of that instance alone, so results are bit-identical
- * to serial runs (for n >= 28 in complex64 compare with the tuning key
- * "fused" = 0: bk_evolve's two-pass runs round differently).  n must be at
- * least the LOW tile (13 for BK_C64, 12 for BK_C128). */
+ * touched).  Each instance runs the same schedule, kernels and arithmetic as
+ * a bk_evolve of that instance alone (fused runs where bk_evolve fuses them,
+ * under the same tuning), so results are bit-identical to serial runs.  n must
+ * be at least the LOW tile (13 for BK_C64, 12 for BK_C128). */
 typedef struct {
     int n;
     uint64_t start_bits;
@@ -198,6 +197,26 @@ int bk_hv(bk_ctx* ctx, int n, double g, int normalize_by_n, int is_complex, cons
  * number of pairs (>= 0) or an error code. */
 int bk_lanczos(bk_ctx* ctx, int n, double g, int normalize_by_n, int max_krylov_dim, double tol,
                int k_pairs, double* evals, double* evecs);
+
+/* ---- the subspace ("sampled-Krylov") solvers and dense ED of diagonalizer.cuh ----
+ * H restricted to the states of basis (host array of dim distinct states < 2^n,
+ * any order, n <= 64): E(s)/n (or E(s)) on the diagonal, -g between states one
+ * spin flip apart.  evals: k_pairs ascending; evecs (host, k_pairs * dim, may be
+ * NULL): each eigenvector in the order of basis.  Return the number of pairs.
+ *
+ * Diagonalizer::solve(basis, g, n, cfg), DiagMethod::Lanczos
+ * (solve_subspace_lanczos, diagonalizer.cuh:299-372): the Lanczos of bk_lanczos
+ * over H.v of the sorted basis (subspace_matvec_kernel_d, :79-114). */
+int bk_subspace_lanczos(bk_ctx* ctx, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim,
+                        int max_krylov_dim, double tol, int k_pairs, double* evals, double* evecs);
+/* DiagMethod::ExactDense on a basis (solve_subspace_dense, diagonalizer.cuh:188-245):
+ * all eigenpairs of the dense H on the device (dim <= 32768). */
+int bk_subspace_dense(bk_ctx* ctx, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim,
+                      int k_pairs, double* evals, double* evecs);
+/* DiagMethod::ExactDense on the full 2^n space (solve_fullspace_dense,
+ * diagonalizer.cuh:250-294): n <= 14 as in the reference; evecs k_pairs * 2^n. */
+int bk_fullspace_dense(bk_ctx* ctx, int n, double g, int normalize_by_n, int k_pairs, double* evals,
+                       double* evecs);
 
 /* ---- Krylov propagator: held complex128 state <- exp(-i H tau) state, m-step
  * Lanczos with full reorthogonalisation, restarted every tau_restart. ---- */

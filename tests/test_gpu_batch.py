@@ -26,7 +26,9 @@ def _serial(bk, engine, inst, dt, steps, order, prec):
 
 
 @pytest.mark.parametrize("prec,ns", [("c64", [13, 14, 15, 16, 17, 16, 14, 13, 18]),
-                                     ("c128", [12, 13, 14, 15, 13, 12])])
+                                     ("c128", [12, 13, 14, 15, 13, 12]),
+                                     ("c64", [24, 25, 24]),    # two high-qubit groups
+                                     ("c128", [22, 23])])
 @pytest.mark.parametrize("order", ["strang", "lie"])
 def test_batch_bit_identical_to_serial(bk, engine, prec, ns, order):
     precision = bk.BK_C64 if prec == "c64" else bk.BK_C128
