@@ -116,8 +116,14 @@ def test_sweep_plan_covers_every_qubit_once(bk, prec):
 _CXX_PROBE = r"""
 #include "bk/sampler.hpp"
 #include "bk/diagonalizer.hpp"
+#include "bk/wire.hpp"
 #include <cstdio>
 int main() {
+    // both Diagonalizer::solve overloads and the connector writers instantiate
+    auto sub = static_cast<Eigensolution (Diagonalizer::*)(const std::vector<SpinState>&, double, int, DiagConfig)
+                               const>(&Diagonalizer::solve);
+    (void)sub;
+    (void)&bk::wire::export_structured_json;
     SamplerConfig c;
     std::printf("cfg %d %g %g %g %d %d %llu\n", c.n, c.g, c.dt, c.t_max, c.num_sample_points,
                 c.shots_per_point, (unsigned long long)c.seed);
