@@ -117,6 +117,11 @@ class _SamplingResultC(C.Structure):
                 ("samples", C.POINTER(C.c_int64)), ("samples_len", C.c_int64)]
 
 
+class _MinimumFamilyC(C.Structure):
+    _fields_ = [("id", C.c_int64), ("canonical_state", C.c_uint64), ("sample_state", C.c_uint64),
+                ("energy", C.c_int64), ("times_visited", C.c_uint64), ("evals_to_first_find", C.c_uint64)]
+
+
 class _TrapRecordC(C.Structure):
     _fields_ = [("n", C.c_int), ("g", C.c_double), ("trap_rank", C.c_int64), ("trap_bits", C.c_uint64),
                 ("canonical_bits", C.c_uint64), ("trap_energy", C.c_int64), ("d_min", C.c_int),
@@ -180,6 +185,12 @@ _SIGS = {
                                     C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bk_fullspace_dense": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
+    "bk_canonical": (C.c_uint64, [C.c_int, C.c_uint64]),
+    "bk_descent_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.c_int64, C.c_uint64,
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "bk_collect_minima": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
+                                    C.POINTER(_MinimumFamilyC), C.c_int64, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "bk_export_results_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
     "bk_export_time_series_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
     "bk_export_structured_json": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int, C.c_int]),
@@ -246,6 +257,11 @@ def d_min(n: int, s0: int, target: int = -1, max_d: int = -1) -> int:
 
 def known_optimal_energy(n: int) -> int:
     return lib().bk_known_optimal_energy(n)
+
+
+def canonical(n: int, s: int) -> int:
+    """SpinState::canonical (labs_energy.hpp:51-78)."""
+    return lib().bk_canonical(n, s)
 
 
 def make_trap(n: int, seed: int):
@@ -517,6 +533,31 @@ class Engine:
         k = _check(lib().bk_lanczos(self._h, n, g, 1 if normalize_by_n else 0, max_krylov_dim,
                                     tol, k_pairs, _dp(ev), _dp(vec) if vectors else None))
         return ev[:k], (vec[:k] if vectors else None)
+
+    def descent_batch(self, n: int, starts, max_evals: int = 0):
+        """Steepest descent of every start on the GPU (classical.cuh:459-535):
+        (final states, final energies, evaluation counts)."""
+        st = np.ascontiguousarray(np.asarray(starts, dtype=np.uint64))
+        k = st.shape[0]
+        fin = np.zeros(k, np.uint64)
+        fe = np.zeros(k, np.int64)
+        ev = np.zeros(k, np.int64)
+        u64 = C.POINTER(C.c_uint64)
+        i64 = C.POINTER(C.c_int64)
+        _check(lib().bk_descent_batch(self._h, n, st.ctypes.data_as(u64), k, max_evals, fin.ctypes.data_as(u64),
+                                      fe.ctypes.data_as(i64), ev.ctypes.data_as(i64)))
+        return fin, fe, ev
+
+    def collect_minima(self, n: int, seed: int, target_families: int, batch_size: int = 4096,
+                       max_starts: int = 100000) -> dict:
+        """collect_distinct_minima_gpu (classical.cuh:861-950): distinct symmetry
+        families of local minima, deepest first."""
+        out = (_MinimumFamilyC * target_families)()
+        cnt, starts, evals = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().bk_collect_minima(self._h, n, seed, target_families, batch_size, max_starts, out,
+                                       target_families, C.byref(cnt), C.byref(starts), C.byref(evals)))
+        fams = [{f: getattr(out[i], f) for f, _ in _MinimumFamilyC._fields_} for i in range(cnt.value)]
+        return {"families": fams, "total_starts": starts.value, "total_evals": evals.value}
 
     def _basis(self, basis) -> np.ndarray:
         return np.ascontiguousarray(np.asarray(basis, dtype=np.uint64))

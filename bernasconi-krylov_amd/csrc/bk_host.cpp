@@ -21,6 +21,21 @@ int64_t host_energy(int n, uint64_t s)
     return e;
 }
 
+uint64_t host_canonical(int n, uint64_t s)
+{
+    const uint64_t mask = low_mask(n);
+    const auto reverse = [n](uint64_t x) {
+        uint64_t r = 0;
+        for (int i = 0; i < n; ++i) r |= ((x >> i) & 1ull) << (n - 1 - i);
+        return r;
+    };
+    // the order-8 group: {identity, alternating flip} x {identity, reversal} x {identity, negation}
+    const uint64_t a = s & mask, b = a ^ (0x5555555555555555ull & mask), c = reverse(a), d = reverse(b);
+    uint64_t best = a;
+    for (uint64_t x : {b, c, d, ~a & mask, ~b & mask, ~c & mask, ~d & mask}) best = x < best ? x : best;
+    return best;
+}
+
 int64_t host_known_optimal(int n)
 {
     static const int64_t tab[33] = {-1, -1, -1, 1,  2,  2,  7,  3,  8,  12, 13, 5,  10, 6,  19, 15, 24,

@@ -10,6 +10,8 @@ namespace bk {
 
 // labs_energy.hpp:121-132
 int64_t host_energy(int n, uint64_t s);
+// SpinState::canonical (labs_energy.hpp:51-78): smallest image under the order-8 symmetry group
+uint64_t host_canonical(int n, uint64_t s);
 // labs_energy.hpp:156-168
 int64_t host_known_optimal(int n);
 // labs_energy.hpp:175-207

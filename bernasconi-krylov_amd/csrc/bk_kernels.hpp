@@ -258,6 +258,11 @@ hipError_t launch_sub_hv(const double* diag, const int32_t* nbr, int64_t dim, in
 hipError_t launch_sub_dense(const double* diag, const int32_t* nbr, int64_t dim, int n, double g, double* H,
                             hipStream_t stream);
 
+// ---- trap supply (classical.cuh:459-535): single-flip steepest descent of
+// count starts, one wave per start; max_evals = 0: no budget ----
+hipError_t launch_descent(int n, int64_t count, const uint64_t* starts, uint64_t max_evals, uint64_t* fin, int64_t* fe,
+                          int64_t* evc, hipStream_t stream);
+
 // plain float4 copy of 'bytes' (multiple of 16): HBM ceiling probe
 hipError_t launch_copy_probe(const void* src, void* dst, uint64_t bytes, hipStream_t stream);
 

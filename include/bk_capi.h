@@ -64,6 +64,41 @@ int64_t bk_known_optimal_energy(int n);
 /* classical.cuh:602-604 (first draw of mt19937_64(seed)) + :625-713 (steepest descent) */
 int bk_make_trap(int n, uint64_t seed, uint64_t* start, uint64_t* s0, int64_t* e0);
 
+/* SpinState::canonical (labs_energy.hpp:51-78): the smallest of the 8 images of s
+ * under reversal, alternating-spin flip and global negation */
+uint64_t bk_canonical(int n, uint64_t s);
+
+/* ---- trap supply at scale (classical.cuh) ----
+ * batch_steepest_descent_kernel (classical.cuh:459-535) / steepest_descent
+ * (:625-713) on the GPU: best-improvement single flips, lowest index on ties,
+ * from each start until no flip lowers E or the next pass would exceed
+ * max_evals_per_start evaluations (0: no budget).  Host arrays of count
+ * entries; eval_counts (may be NULL) counts 1 + n per pass as the reference
+ * does.  Bit-identical to the host descent of bk_make_trap.  2 <= n <= 64. */
+int bk_descent_batch(bk_ctx* ctx, int n, const uint64_t* starts, int64_t count, uint64_t max_evals_per_start,
+                     uint64_t* final_states, int64_t* final_energies, int64_t* eval_counts);
+
+/* MinimumFamily (classical.cuh:60-67) */
+typedef struct {
+    int64_t id;                   /* 1-based rank after sorting */
+    uint64_t canonical_state;     /* canonical representative of the symmetry orbit */
+    uint64_t sample_state;        /* the first converged state found in it */
+    int64_t energy;
+    uint64_t times_visited;       /* descents that ended in the orbit */
+    uint64_t evals_to_first_find; /* evaluations spent until its first discovery */
+} bk_minimum_family;
+
+/* collect_distinct_minima_gpu (classical.cuh:861-950): batches of batch_size
+ * starts drawn from mt19937_64(seed) (random_state, :602-604), descended on the
+ * GPU and grouped by canonical state, until target_families orbits, max_starts
+ * starts or 4 batches without a new orbit; sorted by energy, then visits (then
+ * canonical bits, which the reference leaves unspecified).  Writes up to cap
+ * families to out, their number to *count (total_starts / total_evals may be
+ * NULL). */
+int bk_collect_minima(bk_ctx* ctx, int n, uint64_t seed, int64_t target_families, int64_t batch_size,
+                      int64_t max_starts, bk_minimum_family* out, int64_t cap, int64_t* count, int64_t* total_starts,
+                      int64_t* total_evals);
+
 /* ---- statevector held by the context ---- */
 /* allocate 2^n amplitudes in 'precision' and set |start_bits> (sampler.cuh:428-438) */
 int bk_state_init(bk_ctx* ctx, int n, int precision, uint64_t start_bits);

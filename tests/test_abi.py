@@ -61,6 +61,21 @@ def test_host_math_matches_oracle(bk, orc):
         assert bk.make_trap(n, seed) == orc.make_trap(n, seed)
 
 
+def test_canonical_matches_oracle(bk, orc):
+    """SpinState::canonical: the oracle's (pinned by the reference's fixtures)
+    and the group's defining properties: idempotent, constant on an orbit."""
+    rng = random.Random(23)
+    for _ in range(3000):
+        n = rng.randint(2, 64)
+        s = rng.getrandbits(n)
+        c = bk.canonical(n, s)
+        assert c == orc.canonical(n, s)
+        assert bk.canonical(n, c) == c and c <= s
+        mask = (1 << n) - 1
+        assert bk.canonical(n, ~s & mask) == c
+        assert bk.canonical(n, int(format(s, f"0{n}b")[::-1], 2)) == c
+
+
 def test_trap_with_dmin_filter(bk, orc):
     """Config-1 trap rule: iterate seeds until exact d_min >= 6 (N=24)."""
     seed, start, s0, e0, dm = bk.make_trap_min_dmin(24, 1, 6)
