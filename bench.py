@@ -166,6 +166,36 @@ def _batch_bench(bk, eng, seed: int) -> dict:
     }
 
 
+def _trap_supply_bench(bk, eng, seed: int) -> dict:
+    """Trap supply: steepest descent of 2^18 random N=30 starts on the GPU
+    (bk_descent_batch), beside the oracle's descent (one host thread) on the
+    first 2000 of the same starts, which must agree bit for bit."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # CPU restatement: checker and timed baseline only
+    n, count = 30, 1 << 18
+    starts = np.random.default_rng(seed).integers(0, 1 << n, size=count, dtype=np.uint64)
+    eng.descent_batch(n, starts[:4096])  # warm
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fin, fe, ev = eng.descent_batch(n, starts)
+    t_gpu = time.perf_counter() - t0
+    k = 2000
+    t0 = time.perf_counter()
+    ref = [O.steepest_descent(n, int(s)) for s in starts[:k]]
+    t_cpu = time.perf_counter() - t0
+    same = all((int(fin[i]), int(fe[i]), int(ev[i])) == ref[i][:3] for i in range(k))
+    return {
+        "workload": f"steepest descent of {count} random N={n} starts (numpy default_rng seed {seed})",
+        "gpu_descents_per_s": count / t_gpu,
+        "gpu_energy_evals_per_s": float(ev.sum()) / t_gpu,
+        "cpu_descents_per_s": k / t_cpu,
+        "cpu_sample": f"oracle descent, 1 thread, first {k} starts",
+        "bit_identical_on_sample": same,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +214,8 @@ def main() -> int:
                     help="skip the N=26 adjoint/forward timing")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 512-trap batched timing")
+    ap.add_argument("--no-traps", action="store_true",
+                    help="skip the trap-supply (batched steepest descent) timing")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 8 GiB device copy (HBM ceiling / PMC calibration)")
     ap.add_argument("--cache-sweep", action="store_true",
@@ -260,6 +292,11 @@ def main() -> int:
         print("[bench] batched 512 traps", file=sys.stderr, flush=True)
         batch = _batch_bench(bk, eng, args.seed)
 
+    traps = None
+    if not args.no_traps:
+        print("[bench] trap supply N=30", file=sys.stderr, flush=True)
+        traps = _trap_supply_bench(bk, eng, args.seed)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -309,6 +346,7 @@ def main() -> int:
         },
         "adjoint": adj,
         "batched": batch,
+        "trap_supply": traps,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
