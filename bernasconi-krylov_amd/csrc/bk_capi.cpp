@@ -774,6 +774,17 @@ int bk_adjoint_backward(bk_ctx* c, double w_pless, double w_dh, double* grad_ste
     if ((rc = get_phase_tab(c, n, prec, t.theta - c->theta_pre - t.dt, &tab_init))) return rc;
     if ((rc = get_phase_tab(c, n, prec, -t.dt, &tab_rev))) return rc;
     rs.phase_tab = tab_rev;
+    if (prec == BK_C64 && rs.ecache_high) {
+        // the fused complex64 runs fold each step's c^n into this table; its
+        // carry restarts with every backward so that repeated calls agree bit for bit
+        bk_ctx::FbTab* fb = nullptr;
+        if ((rc = get_fb_tab(c, n, prec, -t.dt, &fb))) return rc;
+        HIPCALL(bk::launch_carry_reset(fb->carry.as<double>(), fb->count, c->stream));
+        rs.fb_applied = fb->applied.p;
+        rs.fb_master = fb->master.as<double>();
+        rs.fb_carry = fb->carry.as<double>();
+        rs.fb_count = fb->count;
+    }
     HIPCALL(bk::launch_adj_init(c->psi.p, c->adj_phi.p, c->adj_chi.p, n, prec, tab_init, static_cast<uint32_t>(t.s0), t.e0,
                                 w_pless, w_dh, c->stream));
     const char* tape = static_cast<const char*>(t.ckpt.p);

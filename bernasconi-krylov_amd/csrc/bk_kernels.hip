@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <numeric>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -167,6 +168,14 @@ struct SweepArgs {
     void* dst2;              // its output (null: in place)
     double bc, bs;     // butterfly coefficients (F_TAN: bs = tan, bc unused)
     double bc2, bs2;   // F_POSTROT: coefficients of the rotations after the phase
+    // F_GRAD with F_TAN: a tan-form rotation leaves both states scaled by 1/c, so
+    // the term of the m-th rotation of a step taken so far is c^-2m too large:
+    // terms taken after m rotations of the step are weighted c^2m.  A tile
+    // sums its terms as T = T * gic + g (gic = 1 / c^2 = 1 + t^2) in rotation
+    // order and adds gw * T, gw = c^2 (m0 + K - 1) for K rotations after m0;
+    // gw2 / gic2 the same for the rotations after the phase.  Horner's form
+    // keeps no power of c live across the tile loop.
+    double gw, gic, gw2, gic2;
     int n;
     // HIGH: the active local bits [R, TB) map to two global segments,
     // [h, h+a1) then [h2, h2+(TB-R-a1)); the tile index fills the remaining
@@ -223,8 +232,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     constexpr int NW = (1 << TT) / 64;  // waves per workgroup
     static_assert(!P.low || TT == kThreadBits, "the LOW sweep runs 256 threads");
     static_assert(NS == 1 || NS == 2, "one or two states");
-    static_assert(NS == 1 || (FLAGS & (F_MEASURE | F_PREFETCH | F_TAN)) == 0,
-                  "two-state sweeps use the (c, s) butterfly without measurement or prefetch");
+    static_assert(NS == 1 || (FLAGS & (F_MEASURE | F_PREFETCH)) == 0,
+                  "two-state sweeps run without measurement or prefetch");
     static_assert((FLAGS & F_GRAD) == 0 || NS == 2, "the gradient pairs two states");
     constexpr bool BATCH = (FLAGS & F_BATCH) != 0;
     constexpr bool POST = (FLAGS & F_POSTROT) != 0;
@@ -422,6 +431,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             if (nxt < a.ntiles) load(tile_base(nxt), ec_base(nxt, tile_base(nxt)), xn, ecn);
         }
         R tg = R(0);
+        [[maybe_unused]] const R gic = static_cast<R>(a.gic);
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             if (p > 0) {
@@ -447,7 +457,11 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
             for (int b = 0; b < RB; ++b) {
                 if (!((P.ph[p].act >> b) & 1u)) continue;
-                if constexpr ((FLAGS & F_GRAD) != 0) tg += grad_terms(b);
+                if constexpr ((FLAGS & F_GRAD) != 0 && (FLAGS & F_TAN) != 0) {
+                    tg = fma(tg, gic, grad_terms(b));
+                } else if constexpr ((FLAGS & F_GRAD) != 0) {
+                    tg += grad_terms(b);
+                }
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -458,7 +472,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                     }
             }
         }
-        if constexpr ((FLAGS & F_GRAD) != 0) gacc[0] += static_cast<double>(tg);
+        if constexpr ((FLAGS & F_GRAD) != 0 && (FLAGS & F_TAN) != 0) gacc[0] += a.gw * static_cast<double>(tg);
+        else if constexpr ((FLAGS & F_GRAD) != 0) gacc[0] += static_cast<double>(tg);
         // after the last rotation of the step: scale (c^n in tan form) and the
         // next step's phase exp(-i dt E/n), then P_< / <d_H> / norm of the
         // stepped state (|psi|^2 does not see the phase)
@@ -525,6 +540,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             // a batched state keeps its field: the next step's coefficients are its own
             const R bs2 = BATCH ? bst : static_cast<R>(a.bs2), bc2 = BATCH ? bct : static_cast<R>(a.bc2);
             R tg2 = R(0);
+            [[maybe_unused]] const R gic2 = static_cast<R>(a.gic2);
 #pragma unroll
             for (int p = NP - 1; p >= 0; --p) {
                 if (p < NP - 1) {
@@ -545,7 +561,11 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
                 for (int b = 0; b < RB; ++b) {
                     if (!((P.ph[p].act >> b) & 1u)) continue;
-                    if constexpr ((FLAGS & F_GRAD) != 0) tg2 += grad_terms(b);
+                    if constexpr ((FLAGS & F_GRAD) != 0 && (FLAGS & F_TAN) != 0) {
+                        tg2 = fma(tg2, gic2, grad_terms(b));
+                    } else if constexpr ((FLAGS & F_GRAD) != 0) {
+                        tg2 += grad_terms(b);
+                    }
 #pragma unroll
                     for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -556,7 +576,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                         }
                 }
             }
-            if constexpr ((FLAGS & F_GRAD) != 0) gacc2[0] += static_cast<double>(tg2);
+            if constexpr ((FLAGS & F_GRAD) != 0 && (FLAGS & F_TAN) != 0) gacc2[0] += a.gw2 * static_cast<double>(tg2);
+            else if constexpr ((FLAGS & F_GRAD) != 0) gacc2[0] += static_cast<double>(tg2);
         }
         if constexpr (STRIDED) {
             // registers are in the last phase's layout, or (POST) in the load layout
@@ -1212,25 +1233,25 @@ hipError_t step_impl(V* psi, V* alt, int* in_alt, const StepSpec& sp, const Swee
     return hipSuccess;
 }
 
-// ---- reverse (adjoint) sweeps: two states, (c, s) butterfly, gradient terms ----
-template <typename V, int R>
+// ---- reverse (adjoint) sweeps: two states, gradient terms ----
+template <typename V, int R, int FL>
 hipError_t launch_high_rev_r(V* phi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
     using PH = HighPlan<C::HTB, C::HRB, R>;
-    auto k = k_sweep<V, PH, F_GRAD, C::SW, 2>;
+    auto k = k_sweep<V, PH, FL, C::SW, 2>;
     hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, phi, a);
     return hipGetLastError();
 }
 
-template <typename V, int R = 3>
+template <typename V, int FL = F_GRAD, int R = 3>
 hipError_t launch_high_rev(V* phi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
 {
     if constexpr (R > Cfg<V>::HTB - 1) {
         return hipErrorInvalidValue;
     } else {
-        if (rbits == R) return launch_high_rev_r<V, R>(phi, a, grid, st);
-        return launch_high_rev<V, R + 1>(phi, rbits, a, grid, st);
+        if (rbits == R) return launch_high_rev_r<V, R, FL>(phi, a, grid, st);
+        return launch_high_rev<V, FL, R + 1>(phi, rbits, a, grid, st);
     }
 }
 
@@ -1607,16 +1628,16 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
 // ---- fused reverse runs (adjoint): the schedule of run_impl on two states ----
 constexpr int kRevPass = F_PHASE | F_GRAD | F_ECACHE | F_POSTROT;
 
-template <typename V, int R = kFusedRMin>
+template <typename V, int FL = kRevPass, int R = kFusedRMin>
 hipError_t launch_high_rev_ppass(V* phi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
     if constexpr (R > kFusedRMax || R >= C::HTB) {
         return hipErrorInvalidValue;
     } else {
-        if (rbits != R) return launch_high_rev_ppass<V, R + 1>(phi, rbits, a, grid, st);
+        if (rbits != R) return launch_high_rev_ppass<V, FL, R + 1>(phi, rbits, a, grid, st);
         using PH = HighPlan<C::HTB, C::HRB, R>;
-        auto k = k_sweep<V, PH, kRevPass, C::SW, 2>;
+        auto k = k_sweep<V, PH, FL, C::SW, 2>;
         hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, phi, a);
         return hipGetLastError();
     }
@@ -1656,16 +1677,43 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
     const bool two = grp.size() == 2;
     const HighSweep A = two ? grp[0] : HighSweep{};
     const HighSweep B = grp.back();
+    // complex64 runs use the tan butterfly when every step allows it: 4 FMAs per
+    // pair instead of 8 operations, with the step's c^n and the rounding
+    // feedback in the phase table and c^2m weights on the gradient terms
+    bool tan = std::is_same_v<V, float2> && rs.fb_applied != nullptr;
+    for (int i = 0; tan && i < K; ++i) tan = mix_coef(n, rs.gdt[i]).tan_form;
     SweepArgs a{};
-    a.tab = rs.phase_tab;
+    a.tab = tan ? rs.fb_applied : rs.phase_tab;
     a.n = n;
     a.psi2 = chi;
     const int gL = clamp_grid(dim >> C::TB, tun.grid), gH = clamp_grid(dim >> C::HTB, tun.grid);
     const int offA = gL, offB = gL + (two ? gH : 0), GS = offB + gH;
     auto region = [&](int i) { return gpart + static_cast<size_t>(i & 1) * static_cast<size_t>(GS); };
+    // coefficients of step i: (c, s), or (unused, t) with c^2 = 1 / (1 + t_f^2)
+    // for the FP32 tangent the kernel uses
     auto coef = [&](int i, double* bc, double* bs) {
-        *bc = std::cos(rs.gdt[i]);
-        *bs = std::sin(rs.gdt[i]);
+        if (tan) {
+            const double t = mix_coef(n, rs.gdt[i]).bs;  // as exact_scale rounds it
+            const double tf = static_cast<double>(static_cast<float>(t));
+            *bc = 1.0 / (1.0 + tf * tf);
+            *bs = t;
+        } else {
+            *bc = std::cos(rs.gdt[i]);
+            *bs = std::sin(rs.gdt[i]);
+        }
+    };
+    // qubits per group and the order the groups of step i are rotated in:
+    // two groups, even i: LOW, A, B; odd i: B, A, LOW; one group: even i: H,
+    // LOW; odd i: LOW, H (the group a phase pass rotates after its phase comes
+    // first in the next step)
+    const int nL = C::TB, nA = two ? A.a1 + A.a2 : 0, nB = B.a1 + B.a2;
+    // weights of a pass doing `count` rotations of step i after `before` of them
+    auto weights = [&](SweepArgs& b, int i, int before, int count) {
+        if (!tan) return;
+        double c2 = 0.0, t = 0.0;
+        coef(i, &c2, &t);
+        b.gw = std::pow(c2, before + count - 1);
+        b.gic = 1.0 + static_cast<double>(static_cast<float>(t)) * static_cast<double>(static_cast<float>(t));
     };
     const double pass_bytes = 4.0 * static_cast<double>(sizeof(V)) * static_cast<double>(dim);
     auto timed = [&](const char* name, auto&& launch) -> hipError_t {
@@ -1674,43 +1722,78 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         if (r == hipSuccess && tun.timer) tun.timer->end(st, name, pass_bytes);
         return r;
     };
+    auto low_rev = [&](bool phase, const SweepArgs& b) -> hipError_t {
+        if constexpr (std::is_same_v<V, float2>) {
+            if (tan) return phase ? launch_low_rev<V, kRevPass | F_TAN>(phi, b, gL, st)
+                                  : launch_low_rev<V, F_GRAD | F_TAN>(phi, b, gL, st);
+        }
+        return phase ? launch_low_rev<V, kRevPass>(phi, b, gL, st) : launch_low_rev<V, F_GRAD>(phi, b, gL, st);
+    };
+    auto high_rev = [&](bool phase, int R, const SweepArgs& b) -> hipError_t {
+        if constexpr (std::is_same_v<V, float2>) {
+            if (tan) return phase ? launch_high_rev_ppass<V, kRevPass | F_TAN>(phi, R, b, gH, st)
+                                  : launch_high_rev<V, F_GRAD | F_TAN>(phi, R, b, gH, st);
+        }
+        return phase ? launch_high_rev_ppass<V>(phi, R, b, gH, st) : launch_high_rev<V>(phi, R, b, gH, st);
+    };
     if (two) {
         SweepArgs b = a;
         b.ntiles = dim >> C::TB;
         coef(0, &b.bc, &b.bs);
+        weights(b, 0, 0, nL);
         b.partials = region(0);
-        e = timed("adjoint_open", [&] { return launch_low_rev<V, F_GRAD>(phi, b, gL, st); });
+        e = timed("adjoint_open", [&] { return low_rev(false, b); });
     } else {
         SweepArgs b = high_args(a, B, dim);
         coef(0, &b.bc, &b.bs);
+        weights(b, 0, 0, nB);
         b.partials = region(0) + offB;
-        e = timed("adjoint_open", [&] { return launch_high_rev<V>(phi, B.R, b, gH, st); });
+        e = timed("adjoint_open", [&] { return high_rev(false, B.R, b); });
     }
     if (e != hipSuccess) return e;
     for (int i = 0; i < K; ++i) {
+        const bool even = (i & 1) == 0;
         if (two) {
             SweepArgs b = high_args(a, A, dim);
             coef(i, &b.bc, &b.bs);
+            weights(b, i, even ? nL : nB, nA);
             b.partials = region(i) + offA;
-            if ((e = timed("adjoint_high_a", [&] { return launch_high_rev<V>(phi, A.R, b, gH, st); })) != hipSuccess)
-                return e;
+            if ((e = timed("adjoint_high_a", [&] { return high_rev(false, A.R, b); })) != hipSuccess) return e;
         }
-        const bool low = two ? (i & 1) != 0 : (i & 1) == 0;
+        if (tan) {
+            // step i's table: P(-dt) times the exact c^n of its rotations, with feedback
+            const int fg = std::max(1, std::min((rs.fb_count + kThreads - 1) / kThreads, 64));
+            hipLaunchKernelGGL(k_phase_feedback<V>, dim3(fg), dim3(kThreads), 0, st,
+                               reinterpret_cast<const double2*>(rs.fb_master), reinterpret_cast<double2*>(rs.fb_carry),
+                               static_cast<V*>(rs.fb_applied), rs.fb_count,
+                               exact_scale<float>(mix_coef(n, rs.gdt[i]), n));
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        const bool low = two ? !even : even;
         SweepArgs b = low ? a : high_args(a, B, dim);
         if (low) b.ntiles = dim >> C::TB;
         b.ecache = low ? rs.ecache : rs.ecache_high;
         coef(i, &b.bc, &b.bs);
+        // the phase pass's group is the last of step i
+        const int cnt = low ? nL : nB;
+        weights(b, i, two ? (even ? nL + nA : nB + nA) : (even ? nB : nL), cnt);
         if (i + 1 < K) {
             coef(i + 1, &b.bc2, &b.bs2);
+            SweepArgs w{};
+            weights(w, i + 1, 0, cnt);  // the group opens step i + 1
+            b.gw2 = w.gw;
+            b.gic2 = w.gic;
         } else {
-            b.bc2 = 1.0;
+            b.bc2 = 1.0;  // identity after the last phase (t = 0 or (c, s) = (1, 0))
             b.bs2 = 0.0;
+            b.gw2 = 1.0;
+            b.gic2 = 1.0;
         }
         const int off = low ? 0 : offB;
         b.partials = region(i) + off;
         b.partials2 = region(i + 1) + off;
-        e = low ? timed("adjoint_low_phase", [&] { return launch_low_rev<V, kRevPass>(phi, b, gL, st); })
-                : timed("adjoint_high_phase", [&] { return launch_high_rev_ppass<V>(phi, B.R, b, gH, st); });
+        e = low ? timed("adjoint_low_phase", [&] { return low_rev(true, b); })
+                : timed("adjoint_high_phase", [&] { return high_rev(true, B.R, b); });
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, region(i), GS, rs.grad_slots[i]);
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -137,6 +137,13 @@ struct RevRunSpec {
     const void* phase_tab;       // exp(-i dtp E/n) per E >> 2
     const uint16_t* ecache;      // LOW-layout E(s)
     const uint16_t* ecache_high; // E(s) in the phase-carrying high group's layout
+    // complex64: an error-feedback table for dtp (as StepSpec's); when given
+    // and every step allows it, the runs use the tan butterfly (the table
+    // carries each step's c^n) instead of (c, s) with phase_tab
+    void* fb_applied = nullptr;
+    const double* fb_master = nullptr;
+    double* fb_carry = nullptr;
+    int fb_count = 0;
 };
 hipError_t launch_reverse_run(void* phi, void* chi, const RevRunSpec& rs, const SweepTuning& tun, double* gpart,
                               hipStream_t stream);

@@ -106,8 +106,10 @@ def test_fused_reverse_matches_step_by_step(bk, n, prec):
     runs end at the checkpoints) against the step-by-step reverse pass, which
     the oracle pins above.  One high group: n = 17 (c128), 23 (c64); two: the
     rest.  50 steps with checkpoints every 16 give runs of 2, 16, 16, 16 steps.
-    complex128: 1e-10 of the largest component; complex64: 1e-4 (the two
-    orders of FP32 rounding, each within 5e-5 of FP64)."""
+    complex128: 1e-10 of the largest component; complex64: 1e-4 (the fused
+    runs use the tan butterfly with c^2m-weighted gradient terms, the
+    step-by-step reverse (c, s); each is within 5e-5 of FP64).  A second
+    backward over the same tape must repeat the gradient bit for bit."""
     precision = bk.BK_C128 if prec == "c128" else bk.BK_C64
     _, s0, e0 = bk.make_trap(n, 7)
     gs = np.repeat(_fields(5, 40 + n), 10)
@@ -119,6 +121,9 @@ def test_fused_reverse_matches_step_by_step(bk, n, prec):
             obs = eng.adjoint_forward(n, s0, gs, 0.02, order=bk.BK_STRANG, precision=precision, e0=e0,
                                       checkpoint_every=16)
             grad, drift = eng.adjoint_backward(1.0, 0.25)
+            # the reverse phase table's rounding carry restarts with each backward
+            again, _ = eng.adjoint_backward(1.0, 0.25)
+            assert np.array_equal(grad, again)
         finally:
             eng.close()
         out[fused] = (np.asarray(obs), grad, drift)
