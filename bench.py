@@ -142,14 +142,15 @@ def _batch_bench(bk, eng, seed: int) -> dict:
         sub = [inst[i] for i in idx]
         eng.batch_run(sub, dt, steps, use_graph=True, autotune=True)
         t0 = time.perf_counter()
-        rs, _ = eng.batch_run(sub, dt, steps, use_graph=True, autotune=True)
+        rs, sst = eng.batch_run(sub, dt, steps, use_graph=True, autotune=True)
         tb = time.perf_counter() - t0
         identical &= bool(np.array_equal(rs, res[idx]))
         step_bytes = len(idx) * float(1 << n) * 18.0  # state read + written (c64) + uint16 energy
         sample[str(n)] = {"instances": len(idx), "serial_s_per_instance": per,
                           "serial_s_estimated": per * len(idx), "batched_s": tb,
                           "batched_over_serial": per * len(idx) / tb,
-                          "batched_hbm_gbs": step_bytes * steps / tb / 1e9}
+                          "batched_hbm_gbs": step_bytes * steps / tb / 1e9,
+                          "block_bytes": sst["block_bytes"], "out_of_place": sst["out_of_place"]}
     return {
         "workload": f"512 traps, n in {{16,18,20,22}} (mt19937_64 seed {seed}), g = 0.10..0.60 (16 values), "
                     f"Strang dt={dt} x {steps} steps, c64, one hipGraph launch",
@@ -161,7 +162,8 @@ def _batch_bench(bk, eng, seed: int) -> dict:
         "batched_over_serial": serial_est / t_batch,
         "per_n": sample,
         "bit_identical_to_serial_sample": identical,
-        "tuning": {"grid": st["grid"], "row_bits": st["row_bits"], "block_bytes": st["block_bytes"]},
+        "tuning": {"grid": st["grid"], "row_bits": st["row_bits"], "block_bytes": st["block_bytes"],
+                   "out_of_place": st["out_of_place"]},
         "graph_nodes": st["launches"],
     }
 

@@ -66,7 +66,8 @@ class Instance(C.Structure):
 
 class BatchParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("order", C.c_int), ("dt", C.c_double), ("steps", C.c_int),
-                ("use_graph", C.c_int), ("autotune", C.c_int)]
+                ("use_graph", C.c_int), ("autotune", C.c_int), ("block_bytes", C.c_int64),
+                ("out_of_place", C.c_int)]
 
 
 class BatchResult(C.Structure):
@@ -75,7 +76,8 @@ class BatchResult(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("grid", C.c_int), ("row_bits", C.c_int),
-                ("block_bytes", C.c_int64), ("launches", C.c_int64), ("tuned", C.c_int)]
+                ("block_bytes", C.c_int64), ("launches", C.c_int64), ("tuned", C.c_int),
+                ("out_of_place", C.c_int)]
 
 
 class Series(C.Structure):
@@ -423,20 +425,25 @@ class Engine:
 
     # ---- batched evolution ----
     def batch_run(self, instances, dt: float, steps: int, order: int = BK_STRANG,
-                  precision: int = BK_C64, use_graph: bool = True, autotune: bool = False):
+                  precision: int = BK_C64, use_graph: bool = True, autotune: bool = False,
+                  block_bytes: int = 0, out_of_place: bool = False):
         """Evolve many traps at once.  instances: iterable of (n, start_bits, e0, g)
-        (e0 < 0 -> E(start_bits)).  Returns (array count x 3 of P_<, <d_H>, norm after
+        (e0 < 0 -> E(start_bits)).  Without autotune, block_bytes > 0 runs the
+        states in cache blocks of that size (each block through all steps), out of
+        place with out_of_place.  Returns (array count x 3 of P_<, <d_H>, norm after
         the last step, stats dict)."""
         inst = list(instances)
         arr = (Instance * len(inst))(*[Instance(int(n), int(s), int(e), float(g)) for n, s, e, g in inst])
         res = (BatchResult * len(inst))()
         st = BatchStats()
         p = BatchParams(precision=precision, order=order, dt=dt, steps=steps,
-                        use_graph=1 if use_graph else 0, autotune=1 if autotune else 0)
+                        use_graph=1 if use_graph else 0, autotune=1 if autotune else 0,
+                        block_bytes=int(block_bytes), out_of_place=1 if out_of_place else 0)
         _check(lib().bk_batch_run(self._h, arr, len(inst), C.byref(p), res, C.byref(st)))
         out = np.array([[r.p_less, r.dh, r.norm] for r in res])
         stats = {"device_ms": st.device_ms, "grid": st.grid, "row_bits": st.row_bits,
-                 "block_bytes": st.block_bytes, "launches": st.launches, "tuned": bool(st.tuned)}
+                 "block_bytes": st.block_bytes, "launches": st.launches, "tuned": bool(st.tuned),
+                 "out_of_place": bool(st.out_of_place)}
         return out, stats
 
     # ---- adjoint gradient ----

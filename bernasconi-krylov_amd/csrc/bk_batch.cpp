@@ -12,6 +12,11 @@
 // form) group).  With a cache block of B bytes, the states are cut into blocks
 // of at most B bytes and each block runs all its steps before the next starts,
 // so a block that fits the 256 MB last-level cache is re-read from there.
+// Out of place, a block's fused passes alternate between its states and a
+// block-sized second buffer: the part's cache serves a copy-shaped pass at
+// close to twice the rate of an in-place update of the same lines, so blocks
+// of a few tens of MB can run above the HBM rate.  Only the observables of the
+// last step leave a run, so the final pass may end in either buffer.
 // The whole sequence can be captured once as a hipGraph and launched once.
 #include <hip/hip_runtime.h>
 
@@ -49,6 +54,7 @@ struct Group {
 struct Tune {
     int grid, row_bits;
     int64_t block_bytes;
+    bool oop;  // blocks of fused groups run out of place
 };
 
 // device pointer of a named arena buffer of at least 'bytes'
@@ -75,9 +81,34 @@ struct Plan {
     uint32_t *s0 = nullptr, *e0 = nullptr;
     double* partials = nullptr;
     double* results = nullptr;
+    char* alt = nullptr;  // out-of-place blocks: the second buffer of a block
     std::map<int, const uint16_t*> ecache;
     size_t total_states_bytes = 0;
 };
+
+// states of group g per block of t
+int block_states(const Plan& P, const Group& g, const Tune& t)
+{
+    const int64_t sb = static_cast<int64_t>(P.esz) << g.n;
+    const int ns = static_cast<int>(g.idx.size());
+    return t.block_bytes <= 0 ? ns : static_cast<int>(std::min<int64_t>(ns, std::max<int64_t>(1, t.block_bytes / sb)));
+}
+
+// the second buffer of out-of-place blocks (one block of the largest fused group)
+int ensure_alt(bk_ctx* c, Plan& P, const Tune& t)
+{
+    if (!t.oop || t.block_bytes <= 0) return BK_OK;
+    size_t need = 0;
+    for (const Group& g : P.groups)
+        if (!g.fused.empty())
+            need = std::max(need, static_cast<size_t>(block_states(P, g, t)) * (P.esz << g.n));
+    if (need == 0) return BK_OK;
+    void* v = nullptr;
+    int rc = arena(c, "alt", need, &v);
+    if (rc) return rc;
+    P.alt = static_cast<char*>(v);
+    return BK_OK;
+}
 
 // enqueue one run of 'steps' steps (starting states prepared by init()) with
 // tuning t; counts the launches
@@ -125,8 +156,17 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
     // one step at a time (classic sweeps), or the fused schedule of launch_run
     auto run_block = [&](const Group& g, int first, int count, int nsteps) -> int {
         int rc = BK_OK;
-        const bk::BatchGroup b = group_block(g, first, count);
+        bk::BatchGroup b = group_block(g, first, count);
         const std::vector<bk::HighSweep>& fg = g.fused;
+        // out of place: each pass writes the other buffer, then the two swap
+        const bool oop = t.oop && t.block_bytes > 0 && !fg.empty() && P.alt;
+        auto pass = [&](const bk::BatchPass& bp, double* part) -> int {
+            if (oop) b.psi_out = b.psi == P.alt ? group_block(g, first, count).psi : P.alt;
+            HIPCALL(bk::launch_batch_pass(b, bp, tun, part, c->stream));
+            if (oop) b.psi = b.psi_out;
+            ++nl;
+            return BK_OK;
+        };
         double* part_last = P.partials + 3 * (g.part_off + static_cast<size_t>(first) * final_tiles(g));
         if (fg.empty()) {
             for (int k = 0; k < nsteps; ++k) {
@@ -141,15 +181,13 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
         bk::BatchPass open{};
         open.kind = two ? bk::BatchPass::kLowPlain : bk::BatchPass::kHighPlain;
         open.hs = fg.back();
-        HIPCALL(bk::launch_batch_pass(b, open, tun, nullptr, c->stream));
-        ++nl;
+        if ((rc = pass(open, nullptr))) return rc;
         for (int k = 0; k < nsteps; ++k) {
             if (two) {
                 bk::BatchPass pa{};
                 pa.kind = bk::BatchPass::kHighPlain;
                 pa.hs = fg[0];
-                HIPCALL(bk::launch_batch_pass(b, pa, tun, nullptr, c->stream));
-                ++nl;
+                if ((rc = pass(pa, nullptr))) return rc;
             }
             if ((rc = feedback(g, first, count))) return rc;
             const bool low = two ? (k & 1) != 0 : (k & 1) == 0;
@@ -160,8 +198,7 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
             pp.ecache_high = g.ecache_hi;
             pp.measure = last;
             pp.post = !last;
-            HIPCALL(bk::launch_batch_pass(b, pp, tun, last ? part_last : nullptr, c->stream));
-            ++nl;
+            if ((rc = pass(pp, last ? part_last : nullptr))) return rc;
         }
         return BK_OK;
     };
@@ -184,8 +221,7 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
         }
     } else {
         for (const Group& g : P.groups) {
-            const int64_t sb = static_cast<int64_t>(P.esz) << g.n;
-            const int per = static_cast<int>(std::max<int64_t>(1, t.block_bytes / sb));
+            const int per = block_states(P, g, t);
             const int ns = static_cast<int>(g.idx.size());
             for (int first = 0; first < ns; first += per)
                 if ((rc = run_block(g, first, std::min(per, ns - first), steps))) return rc;
@@ -233,6 +269,7 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
     if (p->precision != BK_C64 && p->precision != BK_C128) return fail(BK_E_INVALID, "precision=%d", p->precision);
     if (p->order != BK_LIE && p->order != BK_STRANG) return fail(BK_E_INVALID, "order=%d", p->order);
     if (p->steps < 1 || !is_fin(p->dt)) return fail(BK_E_INVALID, "steps=%d dt=%g", p->steps, p->dt);
+    if (p->block_bytes < 0) return fail(BK_E_INVALID, "block_bytes=%lld", static_cast<long long>(p->block_bytes));
     const int prec = p->precision;
     const int tb = bk::low_tile_bits(prec);
     for (int i = 0; i < count; ++i)
@@ -353,21 +390,25 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
     HIPCALL(hipStreamSynchronize(c->stream));  // host vectors go out of scope below
 
     // ---- tuning ----
-    Tune t{4096, prec == BK_C64 ? 5 : 4, 0};
+    Tune t{4096, prec == BK_C64 ? 5 : 4, 0, false};
     const std::string key = shape_key(p, P.groups);
     bool tuned = false;
     auto tit = c->batch_tuned.find(key);
-    if (tit != c->batch_tuned.end()) {
-        t = {tit->second.grid, tit->second.row_bits, tit->second.block_bytes};
+    if (!p->autotune) {
+        t.block_bytes = p->block_bytes;
+        t.oop = p->block_bytes > 0 && p->out_of_place != 0;
+    } else if (tit != c->batch_tuned.end()) {
+        t = {tit->second.grid, tit->second.row_bits, tit->second.block_bytes, tit->second.out_of_place};
         tuned = true;
-    } else if (p->autotune) {
+    } else {
         hipEvent_t e0, e1;
         HIPCALL(hipEventCreate(&e0));
         HIPCALL(hipEventCreate(&e1));
         const int trial_steps = std::min(p->steps, 8);
         auto time_of = [&](const Tune& tt, float* ms) -> int {
-            int r = init_states(c, P);
+            int r = ensure_alt(c, P, tt);
             if (r) return r;
+            if ((r = init_states(c, P))) return r;
             HIPCALL(hipEventRecord(e0, c->stream));
             if ((r = enqueue_run(c, P, trial_steps, tt, nullptr))) return r;
             HIPCALL(hipEventRecord(e1, c->stream));
@@ -393,21 +434,33 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
             if ((rc = time_of(cand, &ms))) return rc;
             if (ms < best) best = ms, t = cand;
         }
+        const Tune base = t;
         for (int64_t bb : {int64_t(256) << 20, int64_t(128) << 20, int64_t(48) << 20, int64_t(8) << 20}) {
             if (bb >= static_cast<int64_t>(P.total_states_bytes)) continue;
-            Tune cand = t;
+            Tune cand = base;
             cand.block_bytes = bb;
+            float ms = 0.f;
+            if ((rc = time_of(cand, &ms))) return rc;
+            if (ms < best) best = ms, t = cand;
+        }
+        // out of place: the block and its second buffer together within the cache
+        for (int64_t bb : {int64_t(128) << 20, int64_t(64) << 20, int64_t(32) << 20, int64_t(16) << 20}) {
+            if (bb >= static_cast<int64_t>(P.total_states_bytes)) continue;
+            Tune cand = base;
+            cand.block_bytes = bb;
+            cand.oop = true;
             float ms = 0.f;
             if ((rc = time_of(cand, &ms))) return rc;
             if (ms < best) best = ms, t = cand;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        c->batch_tuned[key] = {t.grid, t.row_bits, t.block_bytes};
+        c->batch_tuned[key] = {t.grid, t.row_bits, t.block_bytes, t.oop};
         tuned = true;
     }
 
     // ---- run ----
+    if ((rc = ensure_alt(c, P, t))) return rc;
     if ((rc = init_states(c, P))) return rc;
     hipEvent_t ev0, ev1;
     HIPCALL(hipEventCreate(&ev0));
@@ -464,6 +517,7 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
         stats->block_bytes = t.block_bytes;
         stats->launches = launches;
         stats->tuned = tuned ? 1 : 0;
+        stats->out_of_place = t.oop ? 1 : 0;
     }
     return BK_OK;
 }

@@ -181,6 +181,7 @@ struct bk_ctx {
     struct BatchTune {
         int grid = 4096, row_bits = 5;
         int64_t block_bytes = 0;
+        bool out_of_place = false;
     };
     std::map<std::string, BatchTune> batch_tuned;
 };

@@ -416,7 +416,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             inst = static_cast<uint32_t>(tile >> a.inst_shift);
             lt = tile & ((1ull << a.inst_shift) - 1ull);
             sp[0] = psi + (static_cast<uint64_t>(inst) << a.n);
-            dp[0] = sp[0];
+            dp[0] = a.dst ? static_cast<V*>(a.dst) + (static_cast<uint64_t>(inst) << a.n) : sp[0];
             bst = static_cast<R>(a.bs_arr[inst]);
             if constexpr ((FLAGS & F_TAN) == 0) bct = static_cast<R>(a.bc_arr[inst]);
         }
@@ -1378,7 +1378,7 @@ hipError_t batch_step_impl(const BatchGroup& g, const SweepTuning& tun, double* 
 {
     using C = Cfg<V>;
     const int n = g.n;
-    if (n < C::TB || !g.ecache) return hipErrorInvalidValue;
+    if (n < C::TB || !g.ecache || g.psi_out) return hipErrorInvalidValue;  // step-by-step sweeps run in place
     SweepArgs a{};
     a.tab = g.tab;
     a.tab_stride = g.tab_stride;
@@ -1827,6 +1827,7 @@ hipError_t batch_pass_impl(const BatchGroup& g, const BatchPass& bp, const Sweep
     a.s0_arr = g.s0;
     a.e0_arr = g.e0;
     a.partials = partials;
+    a.dst = g.psi_out;
     V* psi = static_cast<V*>(g.psi);
     const bool low = bp.kind == BatchPass::kLowPlain || bp.kind == BatchPass::kLowPhase;
     if (low) {

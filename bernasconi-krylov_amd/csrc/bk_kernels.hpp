@@ -169,6 +169,7 @@ struct BatchGroup {
     int precision;
     int nstates;
     void* psi;                // nstates * 2^n amplitudes, state i at i * 2^n
+    void* psi_out;            // launch_batch_pass: output states in the same layout (null: in place)
     const void* tab;          // error-feedback tables, tab_stride entries per state
     int tab_stride;
     const uint16_t* ecache;   // E(s) cache of n (shared by the states)

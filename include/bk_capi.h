@@ -192,6 +192,9 @@ typedef struct {
     int use_graph;  /* 1: capture the whole run (all steps, all instances) as one hipGraph and launch it once */
     int autotune;   /* 1: choose grid / row bits / cache-block size by timing short trial runs
                        (the choice is cached per workload shape on the context) */
+    int64_t block_bytes; /* autotune = 0: cache block size (0 = every state advances each step) */
+    int out_of_place;    /* autotune = 0, block_bytes > 0: a block's passes ping-pong between its
+                            states and a second block-sized buffer instead of updating in place */
 } bk_batch_params;
 
 typedef struct {
@@ -205,6 +208,7 @@ typedef struct {
     int64_t block_bytes;  /* cache block: states stepped together through all steps (0 = all each step) */
     int64_t launches;     /* kernel launches of the run (graph nodes with use_graph) */
     int tuned;            /* 1: the parameters come from the autotuner */
+    int out_of_place;     /* 1: blocks ran out of place (ping-pong through a block-sized buffer) */
 } bk_batch_stats;
 
 int bk_batch_run(bk_ctx* ctx, const bk_instance* inst, int count, const bk_batch_params* p, bk_batch_result* out,

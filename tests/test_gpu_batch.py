@@ -64,9 +64,31 @@ def test_batch_tuned_blocks_identical(bk, engine):
     assert st["tuned"]
     assert np.array_equal(tuned, ref)
     again, st2 = engine.batch_run(inst, 0.02, 30, precision=bk.BK_C64, use_graph=True, autotune=True)
-    assert st2["tuned"] and (st2["grid"], st2["row_bits"], st2["block_bytes"]) == \
-        (st["grid"], st["row_bits"], st["block_bytes"])
+    assert st2["tuned"] and (st2["grid"], st2["row_bits"], st2["block_bytes"], st2["out_of_place"]) == \
+        (st["grid"], st["row_bits"], st["block_bytes"], st["out_of_place"])
     assert np.array_equal(again, ref)
+
+
+@pytest.mark.parametrize("prec,ns", [("c64", [13, 16, 18, 16, 17, 18, 16, 24]),
+                                     ("c128", [12, 15, 16, 15, 22])])
+@pytest.mark.parametrize("oop", [False, True])
+def test_batch_forced_blocks_bit_identical(bk, engine, prec, ns, oop):
+    """Cache blocks chosen by the caller, in place or out of place (a block's
+    passes alternate between its states and a second buffer), with blocks of
+    one and of several states, a partial last block, and both parities of the
+    pass count (so the last pass ends in either buffer): every instance equals
+    its serial run bit for bit.  Step-by-step groups (c64 n = 13) stay in place."""
+    precision = bk.BK_C64 if prec == "c64" else bk.BK_C128
+    gs = [0.1 + 0.06 * i for i in range(len(ns))]
+    inst = _instances(bk, ns, gs, seed0=120)
+    for steps in (20, 21):
+        ref = _serial(bk, engine, inst, 0.02, steps, bk.BK_STRANG, precision)
+        for bb in (1 << 20, 3 << 20, 64 << 20):
+            for graph in (False, True):
+                got, st = engine.batch_run(inst, 0.02, steps, precision=precision, use_graph=graph,
+                                           block_bytes=bb, out_of_place=oop)
+                assert st["block_bytes"] == bb and st["out_of_place"] == oop and not st["tuned"]
+                assert np.array_equal(got, ref), (steps, bb, graph, np.abs(got - ref).max())
 
 
 def test_batch_rejects_small_n(bk, engine):
