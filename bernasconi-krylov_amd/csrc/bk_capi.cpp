@@ -196,13 +196,15 @@ bool is_fin(double x) { return std::isfinite(x); }
 }  // namespace
 
 // the uint16 E(s) cache of the held state's n, in the LOW-sweep layout of its precision
-int bkc::ensure_ecache(bk_ctx* c)
+int bkc::ensure_ecache(bk_ctx* c, int low_bits)
 {
-    if (c->ecache_n != c->n || c->ecache_prec != c->precision) {
+    const int lb = low_bits > 0 ? low_bits : bk::low_tile_bits(c->precision);
+    if (c->ecache_n != c->n || c->ecache_prec != c->precision || c->ecache_lb != lb) {
         HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
-        HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream));
+        HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream, lb));
         c->ecache_n = c->n;
         c->ecache_prec = c->precision;
+        c->ecache_lb = lb;
     }
     return BK_OK;
 }
@@ -570,9 +572,12 @@ static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, con
 {
     const int K = static_cast<int>(gdt.size());
     if (K == 0) return BK_OK;
-    const std::vector<bk::HighSweep> grp =
-        (c->energy_mode == BK_ENERGY_CACHE16 && !c->tuning.pingpong) ? bk::fused_sweeps(c->n, c->precision, c->tuning)
-                                                                      : std::vector<bk::HighSweep>{};
+    const int lb = bk::fused_low_bits(c->n, c->precision, c->tuning);
+    std::vector<bk::HighSweep> grp;
+    if (c->energy_mode == BK_ENERGY_CACHE16 && !c->tuning.pingpong) grp = bk::fused_sweeps(c->n, c->precision, c->tuning, lb);
+    // a run that would fall back to single steps (a (c, s) step) keeps the default LOW tile
+    for (double x : gdt)
+        if (!bk::mix_info(c->n, x, c->precision).tan_form) grp.clear();
     const bool fusable = !grp.empty();
     if (!fusable) {
         for (int k = 0; k < K; ++k) {
@@ -584,9 +589,10 @@ static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, con
     bk_ctx::FbTab* fb = nullptr;
     int rc = get_fb_tab(c, c->n, c->precision, dt, &fb);
     if (rc) return rc;
-    if ((rc = ensure_ecache(c))) return rc;
+    if ((rc = ensure_ecache(c, lb))) return rc;
     if ((rc = ensure_ecache_high(c, grp.back()))) return rc;
     bk::RunSpec rs{};
+    rs.low_bits = lb;
     rs.n = c->n;
     rs.precision = c->precision;
     rs.steps = K;
@@ -1253,6 +1259,8 @@ int bk_set_tuning(bk_ctx* c, const char* key, int value)
         if (!value) c->psi_alt.release();
     } else if (k == "fused") {
         c->tuning.fused = value != 0;
+    } else if (k == "wide_low") {
+        c->tuning.wide_low = value != 0;
     } else {
         return fail(BK_E_INVALID, "unknown tuning key '%s'", key);
     }

@@ -145,6 +145,7 @@ struct bk_ctx {
     bkc::DevBuf ecache;                        // uint16 E(s) per amplitude (LOW tile layout)
     int ecache_n = 0;
     int ecache_prec = 0;
+    int ecache_lb = 0;                         // LOW tile bits of its layout
     bkc::DevBuf ecache_hi;                     // E(s) in the layout of a fused run's phase-carrying high pass
     std::vector<int> ecache_hi_key;            // (n, precision, h1, a1, h2, a2, R) it was filled for
     bool timing = false;
@@ -193,8 +194,9 @@ int check_ctx(bk_ctx* c);
 bool is_fin(double x);
 // exp(-i dtp E/n) for E = 4m + c_n, m = 0..count-1, in FP64 (re, im pairs)
 std::vector<double> phase_master(int n, double dtp);
-// the uint16 E(s) cache of the held state's n (LOW-sweep layout of its precision)
-int ensure_ecache(bk_ctx* c);
+// the uint16 E(s) cache of the held state's n in the layout of a LOW tile of
+// low_bits (0: the precision's default tile)
+int ensure_ecache(bk_ctx* c, int low_bits = 0);
 // plain Lanczos (diagonalizer.cuh:420-554) over a device matvec on vectors of
 // length dim; returns the number of pairs written to evals (and evecs, host)
 int lanczos_core(bk_ctx* c, int64_t dim, const std::function<hipError_t(const double*, double*)>& matvec,

@@ -201,15 +201,18 @@ __host__ __device__ constexpr int phase_tab_count(int n)
     return static_cast<int>(((static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6) >> 2) + 1);
 }
 
-// complex64 high-qubit passes that apply the phase copy the phase table into
-// LDS behind the 2^14-amplitude exchange buffer (at most 2605 entries, 20.8 KB
-// at n = 32): the per-amplitude table reads become LDS gathers instead of
-// scattered global loads through the vector memory pipeline the state
-// streams use.  Batched passes keep per-state tables in global memory.
-template <typename V, bool LOW, int FLAGS>
+// complex64 passes over 2^14-amplitude tiles (the high-qubit groups and the
+// wide LOW tile) that apply the phase copy the phase table into LDS behind
+// the exchange buffer (at most 2605 entries, 20.8 KB at n = 32): the
+// per-amplitude table reads become LDS gathers instead of scattered global
+// loads through the vector memory pipeline the state streams use.  The
+// 2^13-amplitude LOW tile keeps the table in global memory, where the extra
+// LDS would halve its workgroups per CU.  Batched passes keep per-state
+// tables in global memory.
+template <typename V, bool LOW, int TB, int FLAGS>
 constexpr bool lds_phase_table()
 {
-    return !LOW && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 && sizeof(V) == 8;
+    return (!LOW || TB >= 14) && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 && sizeof(V) == 8;
 }
 
 template <int TB_, int RB_, int R_>
@@ -230,7 +233,6 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     constexpr Plan P = PH::plan;
     constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R, TT = P.TT();
     constexpr int NW = (1 << TT) / 64;  // waves per workgroup
-    static_assert(!P.low || TT == kThreadBits, "the LOW sweep runs 256 threads");
     static_assert(NS == 1 || NS == 2, "one or two states");
     static_assert(NS == 1 || (FLAGS & (F_MEASURE | F_PREFETCH)) == 0,
                   "two-state sweeps run without measurement or prefetch");
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     // the phase table (up to n = 32) behind the exchange buffer.
     // A one-phase plan exchanges nothing: it only needs the reduction
     // scratch (NW x 3 doubles), which leaves its occupancy to the registers.
-    constexpr bool LDS_TAB = lds_phase_table<V, P.low, FLAGS>();
+    constexpr bool LDS_TAB = lds_phase_table<V, P.low, TB, FLAGS>();
     constexpr int XCH = NP > 1 ? 1 << TB : 0;
     constexpr int SCRATCH = (3 * NW * 8 + static_cast<int>(sizeof(V)) - 1) / static_cast<int>(sizeof(V));
     constexpr int LDS_ELEMS = XCH + (LDS_TAB ? phase_tab_count(32) : 0) > SCRATCH
@@ -1003,6 +1005,7 @@ __global__ __launch_bounds__(kThreads) void k_phase_from(V* __restrict__ dst, co
 namespace {
 
 constexpr int kLowTB64 = 13, kLowRB64 = 5;
+constexpr int kWideLowTB = 14;  // complex64 LOW tile of fused runs with 19 high qubits (512 threads, 128 KB LDS)
 constexpr int kLowTB128 = 12, kLowRB128 = 4;
 constexpr int kSW64 = 5, kSW128 = 4;
 
@@ -1429,13 +1432,27 @@ hipError_t batch_step_impl(const BatchGroup& g, const SweepTuning& tun, double* 
 // qubits fit one or two sweep groups (n <= 32)
 constexpr int kFusedRMin = 3, kFusedRMax = 12;
 
-template <typename V, int FL>
+// LTB: tile bits of the LOW pass (the default tile, or the complex64 wide tile
+// of 2^kWideLowTB amplitudes, 512 threads, that a fused run can take)
+template <typename V, int FL, int LTB = Cfg<V>::TB>
 hipError_t launch_low_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, psi, a);
+    using PL = LowPlan<LTB, C::RB>;
+    auto k = k_sweep<V, PL, FL, C::SW>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PL::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
+}
+
+// a LOW pass of a fused run with ltb tile bits
+template <typename V, int FL>
+hipError_t launch_low_run(int ltb, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (std::is_same_v<V, float2>) {
+        if (ltb == kWideLowTB) return launch_low_fl<V, FL, kWideLowTB>(psi, a, grid, st);
+    }
+    if (ltb != Cfg<V>::TB) return hipErrorInvalidValue;
+    return launch_low_fl<V, FL>(psi, a, grid, st);
 }
 
 template <typename V, int R, int FL>
@@ -1462,43 +1479,71 @@ hipError_t launch_highp_r(int rbits, V* psi, const SweepArgs& a, int grid, hipSt
 // phase-carrying passes (tan butterflies): F_PHASE | F_ECACHE | F_TAN | any of
 // F_MEASURE, F_POSTROT
 template <typename V>
-hipError_t launch_ppass(bool low, int fl, int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+hipError_t launch_ppass(bool low, int ltb, int fl, int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     constexpr int B = F_PHASE | F_ECACHE | F_TAN;
     switch (fl) {
-    case B: return low ? launch_low_fl<V, B>(psi, a, grid, st) : launch_highp_r<V, B>(rbits, psi, a, grid, st);
+    case B: return low ? launch_low_run<V, B>(ltb, psi, a, grid, st) : launch_highp_r<V, B>(rbits, psi, a, grid, st);
     case B | F_MEASURE:
-        return low ? launch_low_fl<V, B | F_MEASURE>(psi, a, grid, st)
+        return low ? launch_low_run<V, B | F_MEASURE>(ltb, psi, a, grid, st)
                    : launch_highp_r<V, B | F_MEASURE>(rbits, psi, a, grid, st);
     case B | F_POSTROT:
-        return low ? launch_low_fl<V, B | F_POSTROT>(psi, a, grid, st)
+        return low ? launch_low_run<V, B | F_POSTROT>(ltb, psi, a, grid, st)
                    : launch_highp_r<V, B | F_POSTROT>(rbits, psi, a, grid, st);
     case B | F_MEASURE | F_POSTROT:
-        return low ? launch_low_fl<V, B | F_MEASURE | F_POSTROT>(psi, a, grid, st)
+        return low ? launch_low_run<V, B | F_MEASURE | F_POSTROT>(ltb, psi, a, grid, st)
                    : launch_highp_r<V, B | F_MEASURE | F_POSTROT>(rbits, psi, a, grid, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-// The high-qubit groups of a fused run: the fewest groups (one or two) that
-// the planner makes with rows of at least 2^kFusedRMin amplitudes, the last
-// group (the phase-carrying one) with R in [kFusedRMin, kFusedRMax].
+// The high-qubit groups of a fused run whose LOW pass covers qubits [0, ltb)
+// (0: the default tile): the fewest groups (one or two) that the planner makes
+// with rows of at least 2^kFusedRMin amplitudes, the last group (the
+// phase-carrying one) with R in [kFusedRMin, kFusedRMax].
 template <typename V>
-std::vector<HighSweep> fused_groups(int n, const SweepTuning& tun)
+std::vector<HighSweep> fused_groups(int n, const SweepTuning& tun, int ltb = 0)
 {
     using C = Cfg<V>;
-    if (!tun.fused || n <= C::TB) return {};
+    const int tb = ltb > 0 ? ltb : C::TB;
+    if (!tun.fused || n <= tb) return {};
     for (int rmin = kFusedRMin + 1; rmin >= kFusedRMin; --rmin) {  // prefer 16-amplitude rows
-        std::vector<HighSweep> plan = plan_high_sweeps(n, C::TB, C::HTB, rmin, tun.page_bits);
+        std::vector<HighSweep> plan = plan_high_sweeps(n, tb, C::HTB, rmin, tun.page_bits);
         if (plan.empty() || plan.size() > 2) continue;
         if (plan.back().R < kFusedRMin || plan.back().R > kFusedRMax) continue;
         if (plan.size() == 1) return plan;
         // two groups: both fit with the caller's row preference as well
-        std::vector<HighSweep> pref = plan_high_sweeps(n, C::TB, C::HTB, std::max(tun.row_bits, 3), tun.page_bits);
+        std::vector<HighSweep> pref = plan_high_sweeps(n, tb, C::HTB, std::max(tun.row_bits, 3), tun.page_bits);
         if (pref.size() == 2 && pref.back().R >= kFusedRMin && pref.back().R <= kFusedRMax) return pref;
         return plan;
     }
     return {};
+}
+
+// LOW tile bits of a fused forward run.  complex64 at n = 32 has 19 high
+// qubits: beside a 13-qubit LOW tile two groups of at most 14 - R qubits need
+// 16-amplitude rows (R = 4, 128 B) in one of them, and short rows cost HBM
+// efficiency.  The wide LOW tile (14 qubits) leaves 18 high qubits, two
+// groups of 9 with 32-amplitude rows.
+template <typename V>
+int fused_ltb(int n, const SweepTuning& tun)
+{
+    using C = Cfg<V>;
+    constexpr int kFullRows = 5;  // 32 amplitudes, 256 B
+    if constexpr (std::is_same_v<V, float2>) {
+        if (tun.wide_low) {
+            const std::vector<HighSweep> narrow = fused_groups<V>(n, tun);
+            bool short_rows = false;
+            for (const HighSweep& h : narrow) short_rows |= h.R < kFullRows;
+            if (short_rows && narrow.size() == 2) {
+                const std::vector<HighSweep> wide = fused_groups<V>(n, tun, kWideLowTB);
+                bool ok = wide.size() == 2;
+                for (const HighSweep& h : wide) ok &= h.R >= kFullRows;
+                if (ok) return kWideLowTB;
+            }
+        }
+    }
+    return C::TB;
 }
 
 SweepArgs high_args(const SweepArgs& a, const HighSweep& hs, uint64_t dim)
@@ -1521,11 +1566,14 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
     const int n = rs.n, K = rs.steps;
     const uint64_t dim = 1ull << n;
     hipError_t e = hipSuccess;
-    const std::vector<HighSweep> grp = (K >= 1 && rs.ecache && rs.ecache_high) ? fused_groups<V>(n, tun)
+    const int ltb = rs.low_bits > 0 ? rs.low_bits : C::TB;  // the LOW cache's tile
+    const std::vector<HighSweep> grp = (K >= 1 && rs.ecache && rs.ecache_high) ? fused_groups<V>(n, tun, ltb)
                                                                                : std::vector<HighSweep>{};
     bool fused = !grp.empty();
     for (int k = 0; fused && k < K; ++k)
         if (!mix_coef(n, rs.gdt[k]).tan_form) fused = false;  // phase-carrying passes use the tan butterfly
+    // step by step needs the default LOW cache layout
+    if (!fused && ltb != C::TB && K >= 1) return hipErrorInvalidValue;
     const bool two = grp.size() == 2;
     const HighSweep A = fused && two ? grp[0] : HighSweep{};
     const HighSweep B = fused ? grp.back() : HighSweep{};
@@ -1567,10 +1615,10 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
     if (two) {
         SweepArgs b = a;
         coef(0, &b.bc, &b.bs);
-        b.ntiles = dim >> C::TB;
+        b.ntiles = dim >> ltb;
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
-        e = launch_low_fl<V, F_TAN>(psi, b, g, st);
+        e = launch_low_run<V, F_TAN>(ltb, psi, b, g, st);
         if (e != hipSuccess) return e;
         if (tun.timer) tun.timer->end(st, "fused_low_open", pass_bytes);
     } else {
@@ -1607,14 +1655,14 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         const bool post = k + 1 < K;
         double* slot = rs.slots ? rs.slots[k] : nullptr;
         SweepArgs b = low ? a : high_args(a, B, dim);
-        if (low) b.ntiles = dim >> C::TB;
+        if (low) b.ntiles = dim >> ltb;
         b.ecache = low ? rs.ecache : rs.ecache_high;
         coef(k, &b.bc, &b.bs);
         if (post) coef(k + 1, &b.bc2, &b.bs2);
         const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0);
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
-        e = launch_ppass<V>(low, fl, B.R, psi, b, g, st);
+        e = launch_ppass<V>(low, ltb, fl, B.R, psi, b, g, st);
         if (e != hipSuccess) return e;
         if (tun.timer) tun.timer->end(st, low ? "fused_low_phase" : "fused_high_phase", pass_bytes);
         if (slot) {
@@ -1893,6 +1941,7 @@ SweepTuning default_tuning(int n, int precision)
     t.row_bits = precision == kC64 ? 5 : 4;
     t.grid = 4096;
     t.pingpong = false;
+    t.wide_low = true;
     t.prefetch_low = false;
     t.prefetch_high = false;
     t.page_bits = precision == kC64 ? 20 : 19;
@@ -1977,10 +2026,16 @@ hipError_t launch_phase_from(void* dst, const void* src, int n, int precision, c
     return hipErrorInvalidValue;
 }
 
-std::vector<HighSweep> fused_sweeps(int n, int precision, const SweepTuning& tun)
+int fused_low_bits(int n, int precision, const SweepTuning& tun)
 {
-    if (precision == kC64) return fused_groups<float2>(n, tun);
-    if (precision == kC128) return fused_groups<double2>(n, tun);
+    if (precision == kC64) return fused_ltb<float2>(n, tun);
+    return Cfg<double2>::TB;
+}
+
+std::vector<HighSweep> fused_sweeps(int n, int precision, const SweepTuning& tun, int low_bits)
+{
+    if (precision == kC64) return fused_groups<float2>(n, tun, low_bits);
+    if (precision == kC128) return fused_groups<double2>(n, tun, low_bits);
     return {};
 }
 
@@ -2162,14 +2217,17 @@ hipError_t launch_carry_reset(double* carry, int count, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream)
+hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream, int low_bits)
 {
     const uint64_t dim = 1ull << n;
     const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 65536));
     auto k64 = k_energy_fill<LowPlan<Cfg<float2>::TB, Cfg<float2>::RB>>;
+    auto k64w = k_energy_fill<LowPlan<kWideLowTB, Cfg<float2>::RB>>;
     auto k128 = k_energy_fill<LowPlan<Cfg<double2>::TB, Cfg<double2>::RB>>;
+    const int def = precision == kC64 ? Cfg<float2>::TB : Cfg<double2>::TB;
+    if (low_bits > 0 && low_bits != def && !(precision == kC64 && low_bits == kWideLowTB)) return hipErrorInvalidValue;
     if (precision == kC64)
-        hipLaunchKernelGGL(k64, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
+        hipLaunchKernelGGL(low_bits == kWideLowTB ? k64w : k64, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
     else
         hipLaunchKernelGGL(k128, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
     return hipGetLastError();

@@ -64,6 +64,7 @@ struct SweepTuning {
     bool prefetch_high;  // ... and in the HIGH sweeps
     int page_bits;       // > 0: deal qubits >= page_bits evenly over the HIGH sweeps
     bool fused = true;   // runs of steps use the two-pass schedule when the plan allows
+    bool wide_low = true;  // complex64 fused runs may take a 14-qubit LOW tile (n = 32: full rows in both high groups)
     KernelTimer* timer = nullptr;
 };
 
@@ -102,15 +103,19 @@ struct RunSpec {
     double* fb_carry;
     int fb_count;
     uint32_t s0, e0;
-    const uint16_t* ecache;       // LOW-layout E(s) cache
+    const uint16_t* ecache;       // LOW-layout E(s) cache, filled for low_bits
     const uint16_t* ecache_high;  // E(s) in the phase-carrying high group's layout (launch_energy_fill_high);
                                   // null: step by step
+    int low_bits = 0;             // LOW tile of the fused run (fused_low_bits; 0: the default tile)
 };
 hipError_t launch_run(void* psi, const RunSpec& rs, const SweepTuning& tun, double* partials, hipStream_t stream);
 // the high-qubit groups of a fused run for n: one group (one pass a step),
 // two (two passes a step) or none (runs go step by step); the last group is
 // the phase-carrying one
-std::vector<HighSweep> fused_sweeps(int n, int precision, const SweepTuning& tun);
+// low_bits: the LOW tile the groups are planned around (0: the default tile;
+// fused_low_bits gives the forward run's choice)
+std::vector<HighSweep> fused_sweeps(int n, int precision, const SweepTuning& tun, int low_bits = 0);
+int fused_low_bits(int n, int precision, const SweepTuning& tun);
 hipError_t launch_energy_fill_high(uint16_t* out, int n, int precision, const HighSweep& hs, hipStream_t stream);
 
 // ---- reverse pass of the adjoint gradient ----
@@ -223,7 +228,8 @@ hipError_t launch_carry_reset(double* carry, int count, hipStream_t stream);
 
 // E(s) for all 2^n states (uint16, exact for n <= 32) in the LOW sweep's tile
 // layout for 'precision' (the layout differs between c64 and c128)
-hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream);
+// low_bits: the LOW tile whose layout the cache follows (0: the default tile)
+hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream, int low_bits = 0);
 
 // count of s with E(s) < e0 over all 2^n states (exact, integer)
 hipError_t launch_count_below(int n, uint32_t e0, unsigned long long* d_count,

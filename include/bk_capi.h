@@ -349,10 +349,14 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  *              sweeps instead of giving them all to the last one (0 = contiguous)
  * "prefetch_low" / "prefetch_high" : 1 = issue the next tile's loads before the
  *              current tile's stores (software pipelining) in that sweep kind
- * "fused"    : 1 (default) = a run of K steps whose high qubits form two sweep
- *              groups (n = 28..32 in complex64) takes 2K+1 passes: the pass
- *              that applies step k's phase also rotates its qubits for step k+1;
- *              0 = three or more passes per step
+ * "fused"    : 1 (default) = a run of K steps takes one pass per step when the
+ *              high qubits form one sweep group (complex64 n = 15..23, complex128
+ *              13..21) and 2K+1 passes when they form two (complex64 24..32,
+ *              complex128 22..32): the pass that applies step k's phase also
+ *              rotates its qubits for step k+1; 0 = a step at a time
+ * "wide_low" : 1 (default) = a complex64 fused run whose two high groups would
+ *              need 16-amplitude rows (n = 32) takes a 14-qubit LOW tile instead,
+ *              so both groups keep 32-amplitude rows; 0 = the 13-qubit tile
  * Values persist for the context until changed. */
 int bk_set_tuning(bk_ctx* ctx, const char* key, int value);
 

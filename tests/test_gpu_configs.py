@@ -89,6 +89,30 @@ def test_n32_c64_against_c128(bk):
 
 
 @pytest.mark.slow
+def test_n32_wide_low_tile_matches_narrow_plan(bk):
+    """complex64 N=32 fused runs take a 14-qubit LOW tile, so both high groups
+    keep 32-amplitude rows; the 13-qubit tile's plan (one group with
+    16-amplitude rows) gives the same observables to FP32 rounding."""
+    n = 32
+    _, s0, e0 = bk.make_trap(n, 1)
+    out = {}
+    for wide in (1, 0):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("wide_low", wide)
+            out[wide] = eng.evolve(n, s0, 0.35, 0.02, 12, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                                   measure_every=3)
+        finally:
+            eng.close()
+    a, b = out[1], out[0]
+    print(f"\n[N=32 wide/narrow LOW] max|dP_<| {np.abs(a['p_less'] - b['p_less']).max():.2e} "
+          f"max|d<d_H>| {np.abs(a['dh'] - b['dh']).max():.2e}")
+    assert np.abs(a["p_less"] - b["p_less"]).max() <= 1e-6
+    assert np.abs(a["dh"] - b["dh"]).max() <= 1e-5
+    assert np.abs(a["norm"] - 1.0).max() <= 1e-5
+
+
+@pytest.mark.slow
 def test_n26_trotter_against_krylov(bk, engine):
     n, g, T = 26, 0.35, 10.0
     _, s0, e0 = bk.make_trap(n, 1)
