@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
 
-def _cpu_baseline(n: int, seed: int, threads: int):
+def _cpu_baseline(n: int, seed: int, threads: int, what: str = "same workload"):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # CPU restatement, used only as the timed baseline here
     start, s0, e0 = O.make_trap(n, seed)
@@ -36,7 +36,7 @@ def _cpu_baseline(n: int, seed: int, threads: int):
         "unit": "amp-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"1 Strang step of the N={n} trap (same workload), FP64 complex128 "
+        "sample": f"1 Strang step of the N={n} trap ({what}), FP64 complex128 "
                   f"statevector restatement of sampler.cuh (int energy cache, per-qubit "
                   f"passes, OpenMP {threads} threads); setup excluded; {sec:.2f} s",
     }
@@ -305,6 +305,11 @@ def main() -> int:
         print(f"[bench] cpu baseline N={args.cpu_n} threads={threads}", file=sys.stderr,
               flush=True)
         cpu = _cpu_baseline(args.cpu_n, args.seed, threads)
+        # the same restatement on one core, on a 16x smaller trap of the same construction
+        n1 = max(args.cpu_n - 4, 12)
+        print(f"[bench] cpu baseline N={n1} threads=1", file=sys.stderr, flush=True)
+        one = _cpu_baseline(n1, args.seed, 1, f"the N={args.cpu_n} trap construction at N={n1}")
+        cpu["single_core"] = {"value": one["value"], "unit": one["unit"], "cores": 1, "sample": one["sample"]}
 
     out = {
         "metric": "Trotter amp-updates/s & % HBM peak, N=30 c64; adjoint/forward time ratio",

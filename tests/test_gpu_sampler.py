@@ -102,3 +102,34 @@ def test_projective_samples_match_oracle(bk, orc):
     improved = sum(orc.energy(n, int(x)) < e0 for x in first)
     assert r.improved_shots == improved
     assert sorted(r.unique_basis) == sorted(set(int(x) for x in r.samples))
+
+
+def test_projective_samples_follow_born_rule(bk, orc, engine):
+    """Statistical pin of bk_sample (the reference's sampler is closed, so its
+    draws cannot be matched): 400k shots of an evolved N=10 state against
+    |psi_s|^2 of the oracle's FP64 state -- chi-square goodness of fit (cells
+    with an expected count below 5 pooled) and the Kolmogorov distance of the
+    index CDF at the 0.1 % level.  The shots' uniforms come from a seeded
+    generator, so the outcome is fixed."""
+    from scipy import stats
+    n, g, dt, steps = 10, 0.6, 0.05, 30
+    _, s0, e0 = orc.make_trap(n, 3)
+    engine.evolve(n, s0, g, dt, steps, order=bk.BK_LIE, precision=bk.BK_C128, e0=e0, measure_every=0)
+    p = np.abs(orc.trotter(n, s0, g, dt, steps, order=1, want_state=True)["psi"]) ** 2
+    p = p / p.sum()
+    shots = 400_000
+    idx = engine.sample(np.random.default_rng(2026).random(shots))
+    assert idx.min() >= 0 and idx.max() < (1 << n)
+    counts = np.bincount(idx, minlength=1 << n)
+    expected = p * shots
+    big = expected >= 5.0
+    obs = np.append(counts[big], counts[~big].sum())
+    exp = np.append(expected[big], expected[~big].sum())
+    if exp[-1] == 0.0:
+        obs, exp = obs[:-1], exp[:-1]
+    chi2, pval = stats.chisquare(obs, exp * (obs.sum() / exp.sum()))
+    print(f"\n[born rule n={n}] chi2={chi2:.1f} over {len(obs) - 1} dof, p={pval:.3f}")
+    assert pval > 1e-3
+    ks = np.abs(np.cumsum(counts) / shots - np.cumsum(p)).max()
+    assert ks < 1.95 / np.sqrt(shots)
+
