@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
@@ -218,6 +220,9 @@ def main() -> int:
                     help="skip the 512-trap batched timing")
     ap.add_argument("--no-traps", action="store_true",
                     help="skip the trap-supply (batched steepest descent) timing")
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the complex128 run of the same steps that the complex64 "
+                         "series are compared with")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain 8 GiB device copy (HBM ceiling / PMC calibration)")
     ap.add_argument("--cache-sweep", action="store_true",
@@ -282,6 +287,20 @@ def main() -> int:
                            "gbs": round(r["bytes_per_launch"] / (r["total_ms"] / r["launches"])
                                         * 1e-6, 1)} for r in rows}
 
+    # the same warmup + timed steps in complex128 (FP64): the error of the timed
+    # run's P_< and <d_H> series against it, step by step
+    fp64 = None
+    if prec == bk.BK_C64 and not args.no_fp64_check:
+        print(f"[bench] N={n} c128 reference of the same {args.warmup + args.steps} steps", file=sys.stderr,
+              flush=True)
+        ref = eng.evolve(n, s0, args.g, args.dt, args.warmup + args.steps, order=bk.BK_STRANG,
+                         precision=bk.BK_C128, init=True, measure_every=1)
+        rp, rd = np.asarray(ref["p_less"][args.warmup:]), np.asarray(ref["dh"][args.warmup:])
+        fp64 = {"steps_compared": int(len(rp)),
+                "p_less_max_abs_err": float(np.abs(np.asarray(res["p_less"]) - rp).max()),
+                "dh_max_abs_err": float(np.abs(np.asarray(res["dh"]) - rd).max()),
+                "norm_max_abs_dev": float(np.abs(np.asarray(res["norm"]) - 1.0).max())}
+
     copy_gbs = eng.copy_probe(1 << 33, 3)  # practical read+write ceiling on this device
 
     adj = None
@@ -329,6 +348,7 @@ def main() -> int:
                         f"P_< and <d_H> (FP64) fused every step",
             "n": n, "energy": args.energy, "trap_seed": args.seed, "trap_bits": hex(s0), "E0": e0,
             "p_less_final": float(res["p_less"][-1]), "norm_final": float(res["norm"][-1]),
+            "vs_complex128": fp64,
             "step_hbm_bytes": step_bytes,
             "step_hbm_frac": step_bytes / (step_dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
             # the step's algorithmic bytes as a sweep-per-qubit-group schedule
