@@ -260,8 +260,8 @@ std::string shape_key(const bk_batch_params* p, const std::vector<Group>& groups
 
 extern "C" {
 
-int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_params* p, bk_batch_result* out,
-                 bk_batch_stats* stats)
+static int bk_batch_run_impl(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_params* p,
+                             bk_batch_result* out, bk_batch_stats* stats)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -401,19 +401,19 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
         t = {tit->second.grid, tit->second.row_bits, tit->second.block_bytes, tit->second.out_of_place};
         tuned = true;
     } else {
-        hipEvent_t e0, e1;
-        HIPCALL(hipEventCreate(&e0));
-        HIPCALL(hipEventCreate(&e1));
+        ScopedEvent e0, e1;
+        HIPCALL(e0.create());
+        HIPCALL(e1.create());
         const int trial_steps = std::min(p->steps, 8);
         auto time_of = [&](const Tune& tt, float* ms) -> int {
             int r = ensure_alt(c, P, tt);
             if (r) return r;
             if ((r = init_states(c, P))) return r;
-            HIPCALL(hipEventRecord(e0, c->stream));
+            HIPCALL(hipEventRecord(e0.e, c->stream));
             if ((r = enqueue_run(c, P, trial_steps, tt, nullptr))) return r;
-            HIPCALL(hipEventRecord(e1, c->stream));
-            HIPCALL(hipEventSynchronize(e1));
-            HIPCALL(hipEventElapsedTime(ms, e0, e1));
+            HIPCALL(hipEventRecord(e1.e, c->stream));
+            HIPCALL(hipEventSynchronize(e1.e));
+            HIPCALL(hipEventElapsedTime(ms, e0.e, e1.e));
             return BK_OK;
         };
         float best = 0.f;
@@ -453,8 +453,6 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
             if ((rc = time_of(cand, &ms))) return rc;
             if (ms < best) best = ms, t = cand;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         c->batch_tuned[key] = {t.grid, t.row_bits, t.block_bytes, t.oop};
         tuned = true;
     }
@@ -462,47 +460,49 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
     // ---- run ----
     if ((rc = ensure_alt(c, P, t))) return rc;
     if ((rc = init_states(c, P))) return rc;
-    hipEvent_t ev0, ev1;
-    HIPCALL(hipEventCreate(&ev0));
-    HIPCALL(hipEventCreate(&ev1));
+    ScopedEvent ev0, ev1;
+    HIPCALL(ev0.create());
+    HIPCALL(ev1.create());
     int64_t launches = 0;
     if (p->use_graph) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        struct GraphGuard {  // graph and executable destroyed on every return path, after the stream drains
+            hipStream_t stream;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            ~GraphGuard()
+            {
+                if (exec) {
+                    (void)hipStreamSynchronize(stream);
+                    (void)hipGraphExecDestroy(exec);
+                }
+                if (graph) (void)hipGraphDestroy(graph);
+            }
+        } gg{c->stream};
         HIPCALL(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
         rc = enqueue_run(c, P, p->steps, t, &launches);
-        hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
-        }
+        const hipError_t ce = hipStreamEndCapture(c->stream, &gg.graph);
+        if (rc) return rc;
         HIPCALL(ce);
         size_t nodes = 0;
-        HIPCALL(hipGraphGetNodes(graph, nullptr, &nodes));
+        HIPCALL(hipGraphGetNodes(gg.graph, nullptr, &nodes));
         launches = static_cast<int64_t>(nodes);
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        const hipError_t ie = hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0);
         if (ie != hipSuccess) {
-            (void)hipGraphDestroy(graph);
+            gg.exec = nullptr;
             return fail(BK_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
         }
-        HIPCALL(hipEventRecord(ev0, c->stream));
-        hipError_t le = hipGraphLaunch(exec, c->stream);
-        HIPCALL(hipEventRecord(ev1, c->stream));
-        hipError_t se = hipStreamSynchronize(c->stream);
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        HIPCALL(le);
-        HIPCALL(se);
+        HIPCALL(hipEventRecord(ev0.e, c->stream));
+        HIPCALL(hipGraphLaunch(gg.exec, c->stream));
+        HIPCALL(hipEventRecord(ev1.e, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
     } else {
-        HIPCALL(hipEventRecord(ev0, c->stream));
+        HIPCALL(hipEventRecord(ev0.e, c->stream));
         if ((rc = enqueue_run(c, P, p->steps, t, &launches))) return rc;
-        HIPCALL(hipEventRecord(ev1, c->stream));
+        HIPCALL(hipEventRecord(ev1.e, c->stream));
         HIPCALL(hipStreamSynchronize(c->stream));
     }
     float ms = 0.f;
-    HIPCALL(hipEventElapsedTime(&ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    HIPCALL(hipEventElapsedTime(&ms, ev0.e, ev1.e));
     std::vector<double> res(arr_off * 3);
     HIPCALL(hipMemcpy(res.data(), P.results, res.size() * 8, hipMemcpyDeviceToHost));
     for (const Group& g : P.groups)
@@ -520,6 +520,12 @@ int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_p
         stats->out_of_place = t.oop ? 1 : 0;
     }
     return BK_OK;
+}
+
+int bk_batch_run(bk_ctx* c, const bk_instance* inst, int count, const bk_batch_params* p, bk_batch_result* out,
+                 bk_batch_stats* stats)
+{
+    return bkc::guard_abi([&] { return bk_batch_run_impl(c, inst, count, p, out, stats); });
 }
 
 }  // extern "C"

@@ -195,7 +195,9 @@ bool is_fin(double x) { return std::isfinite(x); }
 
 }  // namespace
 
-// the uint16 E(s) cache of the held state's n, in the LOW-sweep layout of its precision
+// the uint16 E(s) cache of the held state's n, in the layout of a LOW tile of
+// low_bits (0: the precision's default tile); refilled when n, precision or
+// the layout change
 int bkc::ensure_ecache(bk_ctx* c, int low_bits)
 {
     const int lb = low_bits > 0 ? low_bits : bk::low_tile_bits(c->precision);
@@ -301,7 +303,7 @@ int bk_device_count(void)
     return n;
 }
 
-int bk_ctx_create(int device, bk_ctx** out)
+static int bk_ctx_create_impl(int device, bk_ctx** out)
 {
     if (!out) return fail(BK_E_INVALID, "null out pointer");
     *out = nullptr;
@@ -322,6 +324,11 @@ int bk_ctx_create(int device, bk_ctx** out)
     return BK_OK;
 }
 
+int bk_ctx_create(int device, bk_ctx** out)
+{
+    return bkc::guard_abi([&] { return bk_ctx_create_impl(device, out); });
+}
+
 void bk_ctx_destroy(bk_ctx* c)
 {
     if (!c) return;
@@ -340,6 +347,7 @@ void bk_ctx_destroy(bk_ctx* c)
     c->blas_partials.release();
     c->blas_out.release();
     c->ecache.release();
+    c->ecache_hi.release();
     c->tape.ckpt.release();
     c->adj_phi.release();
     c->adj_chi.release();
@@ -372,7 +380,7 @@ int bk_make_trap(int n, uint64_t seed, uint64_t* start, uint64_t* s0, int64_t* e
     return BK_OK;
 }
 
-int bk_state_init(bk_ctx* c, int n, int precision, uint64_t start_bits)
+static int bk_state_init_impl(bk_ctx* c, int n, int precision, uint64_t start_bits)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -383,7 +391,12 @@ int bk_state_init(bk_ctx* c, int n, int precision, uint64_t start_bits)
     return BK_OK;
 }
 
-int bk_state_upload(bk_ctx* c, int n, int precision, const void* host)
+int bk_state_init(bk_ctx* c, int n, int precision, uint64_t start_bits)
+{
+    return bkc::guard_abi([&] { return bk_state_init_impl(c, n, precision, start_bits); });
+}
+
+static int bk_state_upload_impl(bk_ctx* c, int n, int precision, const void* host)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -394,7 +407,12 @@ int bk_state_upload(bk_ctx* c, int n, int precision, const void* host)
     return BK_OK;
 }
 
-int bk_state_download(bk_ctx* c, void* host)
+int bk_state_upload(bk_ctx* c, int n, int precision, const void* host)
+{
+    return bkc::guard_abi([&] { return bk_state_upload_impl(c, n, precision, host); });
+}
+
+static int bk_state_download_impl(bk_ctx* c, void* host)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -406,7 +424,12 @@ int bk_state_download(bk_ctx* c, void* host)
     return BK_OK;
 }
 
-int bk_state_device(bk_ctx* c, void** d, void** st, int* n, int* precision)
+int bk_state_download(bk_ctx* c, void* host)
+{
+    return bkc::guard_abi([&] { return bk_state_download_impl(c, host); });
+}
+
+static int bk_state_device_impl(bk_ctx* c, void** d, void** st, int* n, int* precision)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -419,7 +442,12 @@ int bk_state_device(bk_ctx* c, void** d, void** st, int* n, int* precision)
     return BK_OK;
 }
 
-int bk_count_below(bk_ctx* c, int n, int64_t e0, uint64_t* count)
+int bk_state_device(bk_ctx* c, void** d, void** st, int* n, int* precision)
+{
+    return bkc::guard_abi([&] { return bk_state_device_impl(c, d, st, n, precision); });
+}
+
+static int bk_count_below_impl(bk_ctx* c, int n, int64_t e0, uint64_t* count)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -440,7 +468,12 @@ int bk_count_below(bk_ctx* c, int n, int64_t e0, uint64_t* count)
     return BK_OK;
 }
 
-int bk_observables(bk_ctx* c, uint64_t s0, int64_t e0, double* out3)
+int bk_count_below(bk_ctx* c, int n, int64_t e0, uint64_t* count)
+{
+    return bkc::guard_abi([&] { return bk_count_below_impl(c, n, e0, count); });
+}
+
+static int bk_observables_impl(bk_ctx* c, uint64_t s0, int64_t e0, double* out3)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -453,6 +486,11 @@ int bk_observables(bk_ctx* c, uint64_t s0, int64_t e0, double* out3)
     HIPCALL(hipMemcpyAsync(out3, c->series.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCALL(hipStreamSynchronize(c->stream));
     return BK_OK;
+}
+
+int bk_observables(bk_ctx* c, uint64_t s0, int64_t e0, double* out3)
+{
+    return bkc::guard_abi([&] { return bk_observables_impl(c, s0, e0, out3); });
 }
 
 // Make the stored state carry the opening phase of a run's first step
@@ -574,7 +612,8 @@ static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, con
     if (K == 0) return BK_OK;
     const int lb = bk::fused_low_bits(c->n, c->precision, c->tuning);
     std::vector<bk::HighSweep> grp;
-    if (c->energy_mode == BK_ENERGY_CACHE16 && !c->tuning.pingpong) grp = bk::fused_sweeps(c->n, c->precision, c->tuning, lb);
+    if (c->energy_mode == BK_ENERGY_CACHE16 && !c->tuning.pingpong)
+        grp = bk::fused_sweeps(c->n, c->precision, c->tuning, lb);
     // a run that would fall back to single steps (a (c, s) step) keeps the default LOW tile
     for (double x : gdt)
         if (!bk::mix_info(c->n, x, c->precision).tan_form) grp.clear();
@@ -611,7 +650,7 @@ static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, con
     return BK_OK;
 }
 
-int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
+static int bk_evolve_impl(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -676,6 +715,11 @@ int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
     return BK_OK;
 }
 
+int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
+{
+    return bkc::guard_abi([&] { return bk_evolve_impl(c, p, out); });
+}
+
 // ---- adjoint gradient ----
 // Stored states carry P(theta) = exp(-i theta E/n), theta = the opening phase
 // (phi_k = P(theta) psi_k), so one step is phi_k = P(dt) X(g_k dt) phi_{k-1}
@@ -684,7 +728,7 @@ int bk_evolve(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
 // objective's diagonal weight): then phi'_{k-1} = P(-dt) X(-g_k dt) phi'_k,
 // the same for chi', and dL/dg_k = 2 Re <lambda_k| dU_k/dg |psi_{k-1}>
 // = -2 dt Im <chi'_k| sum_q X_q |phi'_k> (dX/dg = i dt sum_q X_q X).
-int bk_adjoint_forward(bk_ctx* c, const bk_adjoint_params* p, double* obs3)
+static int bk_adjoint_forward_impl(bk_ctx* c, const bk_adjoint_params* p, double* obs3)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -746,7 +790,12 @@ int bk_adjoint_forward(bk_ctx* c, const bk_adjoint_params* p, double* obs3)
     return BK_OK;
 }
 
-int bk_adjoint_backward(bk_ctx* c, double w_pless, double w_dh, double* grad_steps, double* drift)
+int bk_adjoint_forward(bk_ctx* c, const bk_adjoint_params* p, double* obs3)
+{
+    return bkc::guard_abi([&] { return bk_adjoint_forward_impl(c, p, obs3); });
+}
+
+static int bk_adjoint_backward_impl(bk_ctx* c, double w_pless, double w_dh, double* grad_steps, double* drift)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -834,15 +883,26 @@ int bk_adjoint_backward(bk_ctx* c, double w_pless, double w_dh, double* grad_ste
     return BK_OK;
 }
 
-int bk_adjoint_grad(bk_ctx* c, const bk_adjoint_params* p, double w_pless, double w_dh, double* obs3,
-                    double* grad_steps, double* drift)
+int bk_adjoint_backward(bk_ctx* c, double w_pless, double w_dh, double* grad_steps, double* drift)
+{
+    return bkc::guard_abi([&] { return bk_adjoint_backward_impl(c, w_pless, w_dh, grad_steps, drift); });
+}
+
+static int bk_adjoint_grad_impl(bk_ctx* c, const bk_adjoint_params* p, double w_pless, double w_dh, double* obs3,
+                                double* grad_steps, double* drift)
 {
     int rc = bk_adjoint_forward(c, p, obs3);
     if (rc) return rc;
     return bk_adjoint_backward(c, w_pless, w_dh, grad_steps, drift);
 }
 
-int bk_sample(bk_ctx* c, const double* randnums, int64_t shots, int64_t* out_idx)
+int bk_adjoint_grad(bk_ctx* c, const bk_adjoint_params* p, double w_pless, double w_dh, double* obs3,
+                    double* grad_steps, double* drift)
+{
+    return bkc::guard_abi([&] { return bk_adjoint_grad_impl(c, p, w_pless, w_dh, obs3, grad_steps, drift); });
+}
+
+static int bk_sample_impl(bk_ctx* c, const double* randnums, int64_t shots, int64_t* out_idx)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -866,8 +926,13 @@ int bk_sample(bk_ctx* c, const double* randnums, int64_t shots, int64_t* out_idx
     return BK_OK;
 }
 
-int bk_hv(bk_ctx* c, int n, double g, int normalize_by_n, int is_complex, const double* d_x, double* d_y,
-          void* stream)
+int bk_sample(bk_ctx* c, const double* randnums, int64_t shots, int64_t* out_idx)
+{
+    return bkc::guard_abi([&] { return bk_sample_impl(c, randnums, shots, out_idx); });
+}
+
+static int bk_hv_impl(bk_ctx* c, int n, double g, int normalize_by_n, int is_complex, const double* d_x, double* d_y,
+                      void* stream)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -879,8 +944,14 @@ int bk_hv(bk_ctx* c, int n, double g, int normalize_by_n, int is_complex, const 
     return BK_OK;
 }
 
-int bk_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, int max_krylov_dim, double tol, int k_pairs,
-               double* evals, double* evecs)
+int bk_hv(bk_ctx* c, int n, double g, int normalize_by_n, int is_complex, const double* d_x, double* d_y,
+          void* stream)
+{
+    return bkc::guard_abi([&] { return bk_hv_impl(c, n, g, normalize_by_n, is_complex, d_x, d_y, stream); });
+}
+
+static int bk_lanczos_impl(bk_ctx* c, int n, double g, int normalize_by_n, int max_krylov_dim, double tol, int k_pairs,
+                           double* evals, double* evecs)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -892,7 +963,14 @@ int bk_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, int max_krylov_di
         max_krylov_dim, tol, k_pairs, evals, evecs);
 }
 
-int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, double tau_restart, int m)
+int bk_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, int max_krylov_dim, double tol, int k_pairs,
+               double* evals, double* evecs)
+{
+    return bkc::guard_abi(
+        [&] { return bk_lanczos_impl(c, n, g, normalize_by_n, max_krylov_dim, tol, k_pairs, evals, evecs); });
+}
+
+static int bk_krylov_propagate_impl(bk_ctx* c, double g, int normalize_by_n, double tau, double tau_restart, int m)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -903,7 +981,7 @@ int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, dou
     ++c->state_gen;
     const int n = c->n;
     const int64_t dim = 1ll << n;
-    DevBuf V, w, coef;
+    ScopedBuf V, w, coef;  // freed on every return path
     HIPCALL(V.ensure(sizeof(double) * 2 * static_cast<size_t>(dim) * m));
     HIPCALL(w.ensure(sizeof(double) * 2 * static_cast<size_t>(dim)));
     HIPCALL(coef.ensure(sizeof(double) * 2 * m));
@@ -974,10 +1052,12 @@ int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, dou
         HIPCALL(bk::launch_combine(dim, 1, mm, dV, coef.as<double>(), psi, c->stream));
         HIPCALL(hipStreamSynchronize(c->stream));
     }
-    V.release();
-    w.release();
-    coef.release();
     return BK_OK;
+}
+
+int bk_krylov_propagate(bk_ctx* c, double g, int normalize_by_n, double tau, double tau_restart, int m)
+{
+    return bkc::guard_abi([&] { return bk_krylov_propagate_impl(c, g, normalize_by_n, tau, tau_restart, m); });
 }
 
 void bk_sampler_config_defaults(bk_sampler_config* cfg)
@@ -1015,7 +1095,7 @@ void bk_result_free(bk_sampling_result* r)
     delete r;
 }
 
-int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits, bk_sampling_result** out)
+static int bk_sampler_run_impl(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits, bk_sampling_result** out)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -1233,7 +1313,12 @@ int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits,
     return BK_OK;
 }
 
-int bk_set_tuning(bk_ctx* c, const char* key, int value)
+int bk_sampler_run(bk_ctx* c, const bk_sampler_config* cfg, uint64_t start_bits, bk_sampling_result** out)
+{
+    return bkc::guard_abi([&] { return bk_sampler_run_impl(c, cfg, start_bits, out); });
+}
+
+static int bk_set_tuning_impl(bk_ctx* c, const char* key, int value)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -1268,7 +1353,12 @@ int bk_set_tuning(bk_ctx* c, const char* key, int value)
     return BK_OK;
 }
 
-int bk_set_energy_mode(bk_ctx* c, int mode)
+int bk_set_tuning(bk_ctx* c, const char* key, int value)
+{
+    return bkc::guard_abi([&] { return bk_set_tuning_impl(c, key, value); });
+}
+
+static int bk_set_energy_mode_impl(bk_ctx* c, int mode)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -1283,7 +1373,12 @@ int bk_set_energy_mode(bk_ctx* c, int mode)
     return BK_OK;
 }
 
-int bk_sweep_plan(int n, int precision, int row_bits, int page_bits, int* out, int cap)
+int bk_set_energy_mode(bk_ctx* c, int mode)
+{
+    return bkc::guard_abi([&] { return bk_set_energy_mode_impl(c, mode); });
+}
+
+static int bk_sweep_plan_impl(int n, int precision, int row_bits, int page_bits, int* out, int cap)
 {
     if (n < 1 || n > BK_MAX_N || (precision != BK_C64 && precision != BK_C128) || (cap > 0 && !out))
         return fail(BK_E_INVALID, "bk_sweep_plan: bad argument");
@@ -1300,34 +1395,40 @@ int bk_sweep_plan(int n, int precision, int row_bits, int page_bits, int* out, i
     return static_cast<int>(pl.size());
 }
 
-int bk_copy_probe(bk_ctx* c, uint64_t bytes, int reps, double* gbs)
+int bk_sweep_plan(int n, int precision, int row_bits, int page_bits, int* out, int cap)
+{
+    return bkc::guard_abi([&] { return bk_sweep_plan_impl(n, precision, row_bits, page_bits, out, cap); });
+}
+
+static int bk_copy_probe_impl(bk_ctx* c, uint64_t bytes, int reps, double* gbs)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
     if (bytes < 16 || bytes % 16 || reps < 1 || !gbs) return fail(BK_E_INVALID, "bk_copy_probe: bad argument");
-    DevBuf a, b;
+    ScopedBuf a, b;
     HIPCALL(a.ensure(bytes));
     HIPCALL(b.ensure(bytes));
     HIPCALL(hipMemsetAsync(a.p, 0, bytes, c->stream));
     HIPCALL(bk::launch_copy_probe(a.p, b.p, bytes, c->stream));  // warm
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCALL(hipEventCreate(&e0));
-    HIPCALL(hipEventCreate(&e1));
-    HIPCALL(hipEventRecord(e0, c->stream));
+    ScopedEvent e0, e1;
+    HIPCALL(e0.create());
+    HIPCALL(e1.create());
+    HIPCALL(hipEventRecord(e0.e, c->stream));
     for (int r = 0; r < reps; ++r) HIPCALL(bk::launch_copy_probe(a.p, b.p, bytes, c->stream));
-    HIPCALL(hipEventRecord(e1, c->stream));
-    HIPCALL(hipEventSynchronize(e1));
+    HIPCALL(hipEventRecord(e1.e, c->stream));
+    HIPCALL(hipEventSynchronize(e1.e));
     float ms = 0.0f;
-    HIPCALL(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCALL(hipEventElapsedTime(&ms, e0.e, e1.e));
     *gbs = 2.0 * static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9;
-    a.release();
-    b.release();
     return BK_OK;
 }
 
-int bk_timing_enable(bk_ctx* c, int enable)
+int bk_copy_probe(bk_ctx* c, uint64_t bytes, int reps, double* gbs)
+{
+    return bkc::guard_abi([&] { return bk_copy_probe_impl(c, bytes, reps, gbs); });
+}
+
+static int bk_timing_enable_impl(bk_ctx* c, int enable)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -1338,7 +1439,12 @@ int bk_timing_enable(bk_ctx* c, int enable)
     return BK_OK;
 }
 
-int bk_timing_read(bk_ctx* c, bk_kernel_timing* out, int cap, int* count)
+int bk_timing_enable(bk_ctx* c, int enable)
+{
+    return bkc::guard_abi([&] { return bk_timing_enable_impl(c, enable); });
+}
+
+static int bk_timing_read_impl(bk_ctx* c, bk_kernel_timing* out, int cap, int* count)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -1369,6 +1475,11 @@ int bk_timing_read(bk_ctx* c, bk_kernel_timing* out, int cap, int* count)
     *count = static_cast<int>(rows.size());
     for (int i = 0; i < cap && i < static_cast<int>(rows.size()); ++i) out[i] = rows[i];
     return BK_OK;
+}
+
+int bk_timing_read(bk_ctx* c, bk_kernel_timing* out, int cap, int* count)
+{
+    return bkc::guard_abi([&] { return bk_timing_read_impl(c, out, cap, count); });
 }
 
 }  // extern "C"

@@ -152,8 +152,8 @@ extern "C" {
 
 uint64_t bk_canonical(int n, uint64_t s) { return bk::host_canonical(n, s); }
 
-int bk_descent_batch(bk_ctx* c, int n, const uint64_t* starts, int64_t count, uint64_t max_evals_per_start,
-                     uint64_t* final_states, int64_t* final_energies, int64_t* eval_counts)
+static int bk_descent_batch_impl(bk_ctx* c, int n, const uint64_t* starts, int64_t count, uint64_t max_evals_per_start,
+                                 uint64_t* final_states, int64_t* final_energies, int64_t* eval_counts)
 {
     int rc = bkc::check_ctx(c);
     if (rc) return rc;
@@ -164,8 +164,17 @@ int bk_descent_batch(bk_ctx* c, int n, const uint64_t* starts, int64_t count, ui
     return run_descents(c, n, starts, count, max_evals_per_start, &b, final_states, final_energies, eval_counts);
 }
 
-int bk_collect_minima(bk_ctx* c, int n, uint64_t seed, int64_t target_families, int64_t batch_size, int64_t max_starts,
-                      bk_minimum_family* out, int64_t cap, int64_t* count, int64_t* total_starts, int64_t* total_evals)
+int bk_descent_batch(bk_ctx* c, int n, const uint64_t* starts, int64_t count, uint64_t max_evals_per_start,
+                     uint64_t* final_states, int64_t* final_energies, int64_t* eval_counts)
+{
+    return bkc::guard_abi(
+        [&] { return bk_descent_batch_impl(c, n, starts, count, max_evals_per_start, final_states, final_energies,
+                                           eval_counts); });
+}
+
+static int bk_collect_minima_impl(bk_ctx* c, int n, uint64_t seed, int64_t target_families, int64_t batch_size,
+                                  int64_t max_starts, bk_minimum_family* out, int64_t cap, int64_t* count,
+                                  int64_t* total_starts, int64_t* total_evals)
 {
     int rc = bkc::check_ctx(c);
     if (rc) return rc;
@@ -226,6 +235,14 @@ int bk_collect_minima(bk_ctx* c, int n, uint64_t seed, int64_t target_families, 
         out[i].evals_to_first_find = f.evals_first;
     }
     return BK_OK;
+}
+
+int bk_collect_minima(bk_ctx* c, int n, uint64_t seed, int64_t target_families, int64_t batch_size, int64_t max_starts,
+                      bk_minimum_family* out, int64_t cap, int64_t* count, int64_t* total_starts, int64_t* total_evals)
+{
+    return bkc::guard_abi(
+        [&] { return bk_collect_minima_impl(c, n, seed, target_families, batch_size, max_starts, out, cap, count,
+                                            total_starts, total_evals); });
 }
 
 }  // extern "C"

@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <functional>
+#include <new>
 #include <map>
 #include <string>
 #include <utility>
@@ -18,6 +20,22 @@ namespace bkc {
 
 // set the thread's last-error message and return code
 int fail(int code, const char* fmt, ...);
+
+// Run an entry point's body with C++ exceptions (host allocation failures,
+// container errors) turned into error codes: no exception crosses the C ABI.
+template <typename F>
+int guard_abi(F&& body)
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(BK_E_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(BK_E_INVALID, "%s", e.what());
+    } catch (...) {
+        return fail(BK_E_INVALID, "unexpected exception");
+    }
+}
 
 #define HIPCALL(expr)                                                                           \
     do {                                                                                        \
@@ -65,6 +83,19 @@ struct ScopedBuf : DevBuf {
     ScopedBuf(const ScopedBuf&) = delete;
     ScopedBuf& operator=(const ScopedBuf&) = delete;
     ~ScopedBuf() { release(); }
+};
+
+// A HIP event destroyed when it goes out of scope.
+struct ScopedEvent {
+    hipEvent_t e = nullptr;
+    ScopedEvent() = default;
+    ScopedEvent(const ScopedEvent&) = delete;
+    ScopedEvent& operator=(const ScopedEvent&) = delete;
+    hipError_t create() { return hipEventCreate(&e); }
+    ~ScopedEvent()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
 };
 
 // HIP-event timer: events are drawn from a pool, recorded around each launch,

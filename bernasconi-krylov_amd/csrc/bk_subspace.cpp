@@ -128,8 +128,8 @@ int dense_solve(bk_ctx* c, int n, double g, const SortedBasis& s, int k_pairs, d
 
 extern "C" {
 
-int bk_subspace_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim,
-                        int max_krylov_dim, double tol, int k_pairs, double* evals, double* evecs)
+static int bk_subspace_lanczos_impl(bk_ctx* c, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim,
+                                    int max_krylov_dim, double tol, int k_pairs, double* evals, double* evecs)
 {
     int rc = bkc::check_ctx(c);
     if (rc) return rc;
@@ -154,8 +154,16 @@ int bk_subspace_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, const ui
     return kp;
 }
 
-int bk_subspace_dense(bk_ctx* c, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim, int k_pairs,
-                      double* evals, double* evecs)
+int bk_subspace_lanczos(bk_ctx* c, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim,
+                        int max_krylov_dim, double tol, int k_pairs, double* evals, double* evecs)
+{
+    return bkc::guard_abi(
+        [&] { return bk_subspace_lanczos_impl(c, n, g, normalize_by_n, basis, dim, max_krylov_dim, tol, k_pairs, evals,
+                                              evecs); });
+}
+
+static int bk_subspace_dense_impl(bk_ctx* c, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim,
+                                  int k_pairs, double* evals, double* evecs)
 {
     int rc = bkc::check_ctx(c);
     if (rc) return rc;
@@ -165,7 +173,15 @@ int bk_subspace_dense(bk_ctx* c, int n, double g, int normalize_by_n, const uint
     return dense_solve(c, n, g, s, k_pairs, evals, evecs);
 }
 
-int bk_fullspace_dense(bk_ctx* c, int n, double g, int normalize_by_n, int k_pairs, double* evals, double* evecs)
+int bk_subspace_dense(bk_ctx* c, int n, double g, int normalize_by_n, const uint64_t* basis, int64_t dim, int k_pairs,
+                      double* evals, double* evecs)
+{
+    return bkc::guard_abi(
+        [&] { return bk_subspace_dense_impl(c, n, g, normalize_by_n, basis, dim, k_pairs, evals, evecs); });
+}
+
+static int bk_fullspace_dense_impl(bk_ctx* c, int n, double g, int normalize_by_n, int k_pairs, double* evals,
+                                   double* evecs)
 {
     int rc = bkc::check_ctx(c);
     if (rc) return rc;
@@ -176,6 +192,11 @@ int bk_fullspace_dense(bk_ctx* c, int n, double g, int normalize_by_n, int k_pai
     SortedBasis s;
     if ((rc = sort_basis(n, all.data(), static_cast<int64_t>(all.size()), normalize_by_n, &s))) return rc;
     return dense_solve(c, n, g, s, k_pairs, evals, evecs);
+}
+
+int bk_fullspace_dense(bk_ctx* c, int n, double g, int normalize_by_n, int k_pairs, double* evals, double* evecs)
+{
+    return bkc::guard_abi([&] { return bk_fullspace_dense_impl(c, n, g, normalize_by_n, k_pairs, evals, evecs); });
 }
 
 }  // extern "C"

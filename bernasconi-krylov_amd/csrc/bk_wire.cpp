@@ -50,7 +50,7 @@ int guarded(const char* path, F f)
 
 extern "C" {
 
-int bk_export_results_csv(const char* path, const bk_trap_record* recs, int count)
+static int bk_export_results_csv_impl(const char* path, const bk_trap_record* recs, int count)
 {
     std::vector<bk::wire::TrapBenchmark> v;
     const int rc = to_records(recs, count, &v);
@@ -58,7 +58,12 @@ int bk_export_results_csv(const char* path, const bk_trap_record* recs, int coun
     return guarded(path, [&] { bk::wire::export_results_csv(path, v); });
 }
 
-int bk_export_time_series_csv(const char* path, const bk_trap_record* recs, int count)
+int bk_export_results_csv(const char* path, const bk_trap_record* recs, int count)
+{
+    return bkc::guard_abi([&] { return bk_export_results_csv_impl(path, recs, count); });
+}
+
+static int bk_export_time_series_csv_impl(const char* path, const bk_trap_record* recs, int count)
 {
     std::vector<bk::wire::TrapBenchmark> v;
     const int rc = to_records(recs, count, &v);
@@ -66,12 +71,22 @@ int bk_export_time_series_csv(const char* path, const bk_trap_record* recs, int 
     return guarded(path, [&] { bk::wire::export_time_series_csv(path, v); });
 }
 
-int bk_export_structured_json(const char* path, const bk_trap_record* recs, int count, int n_override)
+int bk_export_time_series_csv(const char* path, const bk_trap_record* recs, int count)
+{
+    return bkc::guard_abi([&] { return bk_export_time_series_csv_impl(path, recs, count); });
+}
+
+static int bk_export_structured_json_impl(const char* path, const bk_trap_record* recs, int count, int n_override)
 {
     std::vector<bk::wire::TrapBenchmark> v;
     const int rc = to_records(recs, count, &v);
     if (rc) return rc;
     return guarded(path, [&] { bk::wire::export_structured_json(path, v, n_override); });
+}
+
+int bk_export_structured_json(const char* path, const bk_trap_record* recs, int count, int n_override)
+{
+    return bkc::guard_abi([&] { return bk_export_structured_json_impl(path, recs, count, n_override); });
 }
 
 }  // extern "C"
