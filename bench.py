@@ -113,6 +113,24 @@ def _adjoint_bench(bk, eng, seed: int) -> dict:
     }
 
 
+def _adam_bench(bk, eng, seed: int, iters: int = 50, lr: float = 0.005) -> dict:
+    """The differentiation workload's optimisation: Adam on the 64 segment
+    fields of the N=26 c64 adjoint workload (loss -P_<(T)), each iteration one
+    taped forward and one reverse pass through the torch.autograd.Function."""
+    ag = importlib.import_module("bernasconi-krylov_amd.autograd")
+    n, nseg, sps, dt, K = 26, 64, 20, 0.02, 64
+    _, s0, e0 = bk.make_trap(n, seed)
+    g0 = bk.field_schedule(seed, nseg)
+    t0 = time.perf_counter()
+    res = ag.optimize_schedule(eng, n, s0, g0, dt, sps, iters=iters, lr=lr, e0=e0, checkpoint_every=K)
+    sec = time.perf_counter() - t0
+    loss = res["loss"]
+    return {"workload": f"N={n} c64 Strang dt={dt}, {nseg} segments x {sps} steps, loss -P_<(T), "
+                        f"checkpoints every {K}", "iters": iters, "lr": lr, "seconds": sec,
+            "p_less_T_first": float(-loss[0]), "p_less_T_last": float(-loss[-1]),
+            "g_min": float(res["g"].min()), "g_max": float(res["g"].max())}
+
+
 def _batch_bench(bk, eng, seed: int) -> dict:
     """Batched workload: 512 traps, n drawn from {16, 18, 20, 22} (mt19937_64),
     16 fields evenly spaced in [0.10, 0.60], Strang dt=0.02 x 500 steps, c64,
@@ -220,6 +238,8 @@ def main() -> int:
                     help="skip the 512-trap batched timing")
     ap.add_argument("--no-traps", action="store_true",
                     help="skip the trap-supply (batched steepest descent) timing")
+    ap.add_argument("--adam", action="store_true",
+                    help="also run 50 Adam steps on the N=26 adjoint workload's fields")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the complex128 run of the same steps that the complex64 "
                          "series are compared with")
@@ -308,6 +328,11 @@ def main() -> int:
         print("[bench] adjoint N=26", file=sys.stderr, flush=True)
         adj = _adjoint_bench(bk, eng, args.seed)
 
+    adam = None
+    if args.adam:
+        print("[bench] 50 Adam steps N=26", file=sys.stderr, flush=True)
+        adam = _adam_bench(bk, eng, args.seed)
+
     batch = None
     if not args.no_batch:
         print("[bench] batched 512 traps", file=sys.stderr, flush=True)
@@ -372,6 +397,7 @@ def main() -> int:
             "frac_of_copy_ceiling": achieved / copy_gbs,
         },
         "adjoint": adj,
+        "adam": adam,
         "batched": batch,
         "trap_supply": traps,
         "cpu_baseline": cpu,

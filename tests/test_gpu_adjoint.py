@@ -99,6 +99,35 @@ def test_adjoint_matches_fd_of_gpu_forward(bk, engine):
         assert gseg[j] == pytest.approx(fd, abs=1e-4), (j, gseg[j], fd)
 
 
+@pytest.mark.slow
+def test_differentiation_workload_n26_against_fd(bk, engine):
+    """The differentiation workload at full size: N=26 complex64, 64 segments x
+    20 steps of Strang dt=0.02, g_j = 0.35 +- 0.05 from mt19937_64(1), loss
+    -P_<(T), checkpoints every 64 steps.  The adjoint's segment gradient matches
+    central differences (h = 1e-4) of complex128 forward runs to 1e-4 on the
+    first, a middle and the last segment."""
+    n, nseg, sps, dt, K = 26, 64, 20, 0.02, 64
+    _, s0, e0 = bk.make_trap(n, 1)
+    g = bk.field_schedule(1, nseg)
+    engine.adjoint_forward(n, s0, np.repeat(g, sps), dt, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                           checkpoint_every=K)
+    grad, drift = engine.adjoint_backward(-1.0, 0.0)
+    gseg = grad.reshape(nseg, sps).sum(axis=1)
+    assert drift < 1e-9
+    h = 1e-4
+    for j in (0, 31, 63):
+        vals = []
+        for sgn in (1, -1):
+            gg = g.copy()
+            gg[j] += sgn * h
+            r = engine.evolve(n, s0, 0.0, dt, nseg * sps, order=bk.BK_STRANG, precision=bk.BK_C128, e0=e0,
+                              g_steps=np.repeat(gg, sps), measure_every=0)
+            vals.append(-r["p_less"][-1])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        print(f"\n[N=26 segment {j}] adjoint {gseg[j]:+.8e} fd {fd:+.8e}")
+        assert gseg[j] == pytest.approx(fd, abs=1e-4)
+
+
 @pytest.mark.parametrize("n,prec", [(17, "c128"), (22, "c128"), (23, "c64"), (24, "c64"), (25, "c64")])
 def test_fused_reverse_matches_step_by_step(bk, n, prec):
     """Fused reverse runs (the forward run's schedule on both states; the
