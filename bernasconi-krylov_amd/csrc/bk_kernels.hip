@@ -58,6 +58,23 @@ __device__ __forceinline__ uint32_t labs_energy_u32(uint32_t s, int n)
     return e;
 }
 
+// Amplitude store of a sweep.  NT: a nontemporal store, which streams the
+// tile back without keeping it in the caches; measured faster for passes
+// without the phase and for LOW-tile passes over one large state, slower for
+// the phase passes of high-qubit groups and for batched passes, whose small
+// states are re-read from the last-level cache by the next pass.
+template <bool NT, typename V>
+__device__ __forceinline__ void store_amp(V* p, const V& v)
+{
+    if constexpr (NT) {
+        using R = typename RealOf<V>::T;
+        typedef R N __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(*reinterpret_cast<const N*>(&v), reinterpret_cast<N*>(p));
+    } else {
+        *p = v;
+    }
+}
+
 template <typename V, typename R>
 __device__ __forceinline__ void bfly_tan(V& a, V& b, R t)
 {
@@ -243,6 +260,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                   "post-phase rotations: one state (or the gradient's two), with the phase, no prefetch");
     static_assert(!BATCH || (NS == 1 && (FLAGS & F_PREFETCH) == 0), "batched sweeps: one state per tile, no prefetch");
     constexpr bool LOW = P.low;
+    constexpr bool NTSTORE = !BATCH && (LOW || (FLAGS & F_PHASE) == 0);
     V* sp[NS];
     V* dp[NS];
     sp[0] = psi;
@@ -593,7 +611,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             for (int s = 0; s < NS; ++s) {
                 V* op = dp[s] + base + (POST ? tg0 : tgL);
 #pragma unroll
-                for (int j = 0; j < NR; ++j) op[reg_off(str, j)] = x[s][j];
+                for (int j = 0; j < NR; ++j) store_amp<NTSTORE>(op + reg_off(str, j), x[s][j]);
             }
         } else {
 #pragma unroll
@@ -603,7 +621,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
                     const uint32_t pj = reg_pattern(P.ph[SP], RB, j);
-                    op[(POST ? tg0 : tgL) + gl(pj)] = x[s][j];
+                    store_amp<NTSTORE>(op + (POST ? tg0 : tgL) + gl(pj), x[s][j]);
                 }
             }
         }
