@@ -47,3 +47,30 @@ def test_sampler_run_written_like_reference(bk, golden, tmp_path):
     q = j["seeds"][0]["quantum_dynamics"]["0.30"]
     assert q["p_less_array"] == pytest.approx(np.asarray(ref["p_less"]), rel=2e-6, abs=1e-15)
     assert len(q["sampling_validation"]["seeds"]) == cfg.num_sampling_seeds
+
+
+def test_trajectory_run_shape_like_connector_test(bk, tmp_path):
+    """test_connector.cpp Test 5 (lines 104-134): N=6, two traps x g in {0.30,
+    0.50}, trajectory every 0.05 up to t=2, no classical leg -> four records of
+    41 time points each, a time-series CSV of 1 + 4 x 41 = 165 lines, and a
+    results CSV of one row per record."""
+    wire = importlib.import_module("bernasconi-krylov_amd.wire")
+    n, recs = 6, []
+    for rank, seed in enumerate((1, 2), start=1):
+        _, s0, e0 = bk.make_trap(n, seed)
+        for g in (0.30, 0.50):
+            cfg = bk.SamplerConfig(n=n, g=g, dt=0.05, t_max=2.0, record_trajectory=True, dt_trajectory=0.05,
+                                   verbose=False)
+            res = bk.create_sampler(n).sample(s0, cfg)
+            assert len(res.time_points) == 41 and len(res.p_less_history) == 41
+            assert res.p_blind >= 0.0
+            recs.append(wire.TrapRecord(n=n, g=g, trap_rank=rank, trap_bits=s0, canonical_bits=bk.canonical(n, s0),
+                                        trap_energy=e0, result=res, window_t_start=cfg.t_start,
+                                        window_points=cfg.num_sample_points, window_dt=cfg.dt_sample,
+                                        t_max=cfg.t_max))
+    ts = tmp_path / "probability_dynamics_n6.csv"
+    wire.export_time_series_csv(str(ts), recs)
+    assert len(ts.read_text().splitlines()) == 165
+    rs = tmp_path / "results_n6.csv"
+    wire.export_results_csv(str(rs), recs)
+    assert len(rs.read_text().splitlines()) == 1 + len(recs)
