@@ -185,7 +185,7 @@ def test_pingpong_and_grid_bit_identical(engine, orc, bk, n, prec, fused):
         engine.set_tuning("fused", 1)
 
 
-@pytest.mark.parametrize("n", [14, 17, 19, 22, 25, 28, 30])
+@pytest.mark.parametrize("n", [14, 17, 19, 21, 22, 23, 25, 27, 28, 29, 30, 31])
 def test_fused_run_matches_step_by_step(bk, n):
     """Fused runs carry each step's phase inside a pass that also opens the next
     step on its qubits: one high group takes K+1 passes, two take 2K+1, against
