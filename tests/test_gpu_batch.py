@@ -27,6 +27,7 @@ def _serial(bk, engine, inst, dt, steps, order, prec):
 
 @pytest.mark.parametrize("prec,ns", [("c64", [13, 14, 15, 16, 17, 16, 14, 13, 18]),
                                      ("c128", [12, 13, 14, 15, 13, 12]),
+                                     ("c64", [19, 21, 23, 20, 22]),  # one group, upper sizes
                                      ("c64", [24, 25, 24]),    # two high-qubit groups
                                      ("c128", [22, 23])])
 @pytest.mark.parametrize("order", ["strang", "lie"])
