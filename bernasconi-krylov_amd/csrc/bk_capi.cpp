@@ -211,6 +211,30 @@ int bkc::ensure_ecache(bk_ctx* c, int low_bits)
     return BK_OK;
 }
 
+// H.v buffers for n spins: E(s) in state order (null below the tiled sizes,
+// whose kernel computes E itself; filled on the context stream, complete on
+// return) and the scratch vector of the high-qubit passes (null if none is needed)
+static int hv_buffers(bk_ctx* c, int n, int is_complex, const uint16_t** out, double** scratch)
+{
+    *out = nullptr;
+    *scratch = nullptr;
+    const size_t sd = bk::hv_scratch_doubles(n, is_complex);
+    if (sd > 0) {
+        HIPCALL(c->hv_scratch.ensure(sizeof(double) * sd));
+        *scratch = c->hv_scratch.as<double>();
+    }
+    if (n < 13) return BK_OK;
+    if (c->hv_ecache_n != n) {
+        c->hv_ecache_n = 0;
+        HIPCALL(c->hv_ecache.ensure(sizeof(uint16_t) << n));
+        HIPCALL(bk::launch_energy_natural(c->hv_ecache.as<uint16_t>(), n, c->stream));
+        HIPCALL(hipStreamSynchronize(c->stream));
+        c->hv_ecache_n = n;
+    }
+    *out = c->hv_ecache.as<uint16_t>();
+    return BK_OK;
+}
+
 // Plain Lanczos over a device matvec on vectors of length dim
 // (run_gpu_lanczos_internal, diagonalizer.cuh:420-554): v0 from mt19937_64(42),
 // uniform(-0.5, 0.5); no reorthogonalisation; breakdown at |beta| < tol; the
@@ -348,6 +372,8 @@ void bk_ctx_destroy(bk_ctx* c)
     c->blas_out.release();
     c->ecache.release();
     c->ecache_hi.release();
+    c->hv_ecache.release();
+    c->hv_scratch.release();
     c->tape.ckpt.release();
     c->adj_phi.release();
     c->adj_chi.release();
@@ -939,7 +965,10 @@ static int bk_hv_impl(bk_ctx* c, int n, double g, int normalize_by_n, int is_com
     if (n < 1 || n > BK_MAX_N || !d_x || !d_y || d_x == d_y || !is_fin(g))
         return fail(BK_E_INVALID, "bk_hv: bad argument");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    HIPCALL(bk::launch_hv(n, g, normalize_by_n, is_complex, d_x, d_y, st));
+    const uint16_t* ec = nullptr;
+    double* scr = nullptr;
+    if ((rc = hv_buffers(c, n, is_complex ? 1 : 0, &ec, &scr))) return rc;
+    HIPCALL(bk::launch_hv(n, g, normalize_by_n, is_complex, d_x, d_y, st, ec, scr));
     HIPCALL(hipStreamSynchronize(st));
     return BK_OK;
 }
@@ -957,9 +986,12 @@ static int bk_lanczos_impl(bk_ctx* c, int n, double g, int normalize_by_n, int m
     if (rc) return rc;
     if (n < 1 || n > 30 || max_krylov_dim < 1 || k_pairs < 1 || !evals || !is_fin(g))
         return fail(BK_E_INVALID, "bk_lanczos: bad argument");
+    const uint16_t* ec = nullptr;
+    double* scr = nullptr;
+    if ((rc = hv_buffers(c, n, 0, &ec, &scr))) return rc;
     return lanczos_core(
         c, 1ll << n,
-        [&](const double* x, double* y) { return bk::launch_hv(n, g, normalize_by_n, 0, x, y, c->stream); },
+        [&](const double* x, double* y) { return bk::launch_hv(n, g, normalize_by_n, 0, x, y, c->stream, ec, scr); },
         max_krylov_dim, tol, k_pairs, evals, evecs);
 }
 
@@ -981,10 +1013,15 @@ static int bk_krylov_propagate_impl(bk_ctx* c, double g, int normalize_by_n, dou
     ++c->state_gen;
     const int n = c->n;
     const int64_t dim = 1ll << n;
-    ScopedBuf V, w, coef;  // freed on every return path
+    const uint16_t* ec = nullptr;
+    double* scr = nullptr;
+    if ((rc = hv_buffers(c, n, 1, &ec, &scr))) return rc;
+    ScopedBuf V, w, coef, gsc, gsp;  // freed on every return path
     HIPCALL(V.ensure(sizeof(double) * 2 * static_cast<size_t>(dim) * m));
     HIPCALL(w.ensure(sizeof(double) * 2 * static_cast<size_t>(dim)));
     HIPCALL(coef.ensure(sizeof(double) * 2 * m));
+    HIPCALL(gsc.ensure(sizeof(double) * 4 * static_cast<size_t>(bk::kMdMax)));  // coefficients of both sweeps
+    HIPCALL(gsp.ensure(sizeof(double) * bk::multidot_partials_doubles()));
     HIPCALL(c->blas_partials.ensure(sizeof(double) * bk::blas_partials_doubles()));
     HIPCALL(c->blas_out.ensure(sizeof(double) * 4));
     double* dV = V.as<double>();
@@ -1012,18 +1049,27 @@ static int bk_krylov_propagate_impl(bk_ctx* c, double g, int normalize_by_n, dou
         int mm = m;
         for (int j = 0; j < m; ++j) {
             double* vj = dV + static_cast<size_t>(j) * 2 * dim;
-            HIPCALL(bk::launch_hv(n, g, normalize_by_n, 1, vj, dw, c->stream));
+            HIPCALL(bk::launch_hv(n, g, normalize_by_n, 1, vj, dw, c->stream, ec, scr));
+            // classical Gram-Schmidt twice against v_0..v_j, in blocks of at most
+            // kMdMax vectors; the coefficient of v_j in each sweep adds to alpha_j
+            // (each sweep keeps its coefficients in its own region; the last
+            // block of a sweep, which holds v_j, is the one left there)
+            double* dgs = gsc.as<double>();
             for (int pass = 0; pass < 2; ++pass)
-                for (int i = 0; i <= j; ++i) {
-                    double* vi = dV + static_cast<size_t>(i) * 2 * dim;
-                    double cr = 0.0, ci = 0.0;
-                    if ((rc = zdot(vi, dw, &cr, &ci))) return rc;
-                    if (i == j) al[j] += cr;
-                    HIPCALL(bk::launch_axpy(dim, 1, -cr, -ci, vi, dw, c->stream));
+                for (int i0 = 0; i0 <= j; i0 += bk::kMdMax) {
+                    const int mm = std::min(bk::kMdMax, j + 1 - i0);
+                    HIPCALL(bk::launch_gs_block(dim, mm, dV + static_cast<size_t>(i0) * 2 * dim, dw,
+                                                dgs + 2 * bk::kMdMax * pass, gsp.as<double>(), c->stream));
                 }
-            if (j == m - 1) break;
+            const int last = j % bk::kMdMax;  // v_j's position in its block
+            double aj[2] = {0.0, 0.0};
+            for (int pass = 0; pass < 2; ++pass)
+                HIPCALL(hipMemcpyAsync(aj + pass, dgs + 2 * bk::kMdMax * pass + 2 * last, sizeof(double),
+                                       hipMemcpyDeviceToHost, c->stream));
             double bsq = 0.0;
-            if ((rc = zdot(dw, dw, &bsq, nullptr))) return rc;
+            if ((rc = zdot(dw, dw, &bsq, nullptr))) return rc;  // synchronises the stream (aj too)
+            al[j] = aj[0] + aj[1];
+            if (j == m - 1) break;
             const double bj = std::sqrt(bsq);
             if (bj < 1e-13 * (std::fabs(al[j]) + 1.0)) {
                 mm = j + 1;

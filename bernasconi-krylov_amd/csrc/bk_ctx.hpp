@@ -177,6 +177,9 @@ struct bk_ctx {
     int ecache_n = 0;
     int ecache_prec = 0;
     int ecache_lb = 0;                         // LOW tile bits of its layout
+    bkc::DevBuf hv_ecache;                     // uint16 E(s) in state order, for H.v
+    int hv_ecache_n = 0;
+    bkc::DevBuf hv_scratch;                    // one vector for the high-qubit passes of H.v
     bkc::DevBuf ecache_hi;                     // E(s) in the layout of a fused run's phase-carrying high pass
     std::vector<int> ecache_hi_key;            // (n, precision, h1, a1, h2, a2, R) it was filled for
     bool timing = false;

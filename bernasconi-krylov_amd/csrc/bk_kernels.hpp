@@ -244,9 +244,15 @@ hipError_t launch_sample(const void* psi, int n, int precision, const double* d_
 size_t sample_scratch_doubles(int n);
 
 // ---- H.v (matrix-free, diagonalizer.cuh:60-77) ----
-// y = (E/n or E) * x - g * sum_j X_j x ; real FP64 (complex=0) or complex128 (complex=1)
+// y = (E/n or E) * x - g * sum_j X_j x ; real FP64 (complex=0) or complex128 (complex=1).
+// ecache: E(s) as uint16 in state order (launch_energy_natural), required from
+// n = 13 (complex128) / 14 (real) up.  scratch: hv_scratch_doubles(n, is_complex)
+// doubles that let every high-qubit pass write one array instead of updating
+// y (null: they update y).
 hipError_t launch_hv(int n, double g, int normalize_by_n, int is_complex, const double* x,
-                     double* y, hipStream_t stream);
+                     double* y, hipStream_t stream, const uint16_t* ecache = nullptr, double* scratch = nullptr);
+size_t hv_scratch_doubles(int n, int is_complex);
+hipError_t launch_energy_natural(uint16_t* out, int n, hipStream_t stream);
 
 // ---- BLAS-1 with deterministic reductions (FP64) ----
 // out[0] = sum_i x_i * y_i over len elements (complex: sum conj(x_i) y_i -> out[0..1])
@@ -261,6 +267,13 @@ hipError_t launch_scal(int64_t len, double a, double* x, hipStream_t stream);
 hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V,
                           const double* coef, double* y, hipStream_t stream);
 size_t blas_partials_doubles();
+// One classical Gram-Schmidt sweep of complex128 w against m <= kMdMax basis
+// vectors V_0..V_{m-1} (stride len): coef = V^H w (device, 2m doubles, fixed-order
+// reductions), then w -= V coef.  partials: multidot_partials_doubles() doubles.
+constexpr int kMdMax = 32;
+hipError_t launch_gs_block(int64_t len, int m, const double* V, double* w, double* coef, double* partials,
+                           hipStream_t stream);
+size_t multidot_partials_doubles();
 
 // ---- subspace Hamiltonian over a sorted basis (diagonalizer.cuh:79-114, 188-245) ----
 // nbr[i * n + j] = index of basis[i] ^ 2^j in the sorted basis, or -1

@@ -9,6 +9,7 @@
 //    cuBLAS calls of diagonalizer.cuh:449-538).
 #include "bk_kernels.hpp"
 #include "bk_plan.hpp"
+#include "bk_sweeps.hpp"
 
 #include <algorithm>
 
@@ -209,67 +210,187 @@ __device__ __forceinline__ void axpy_(double2& y, double a, const double2& x)
 __device__ __forceinline__ double scl(double a, double x) { return a * x; }
 __device__ __forceinline__ double2 scl(double a, const double2& x) { return make_double2(a * x.x, a * x.y); }
 
+// H.v tiles work on 16-byte units (two FP64 reals, or one complex128): a tile
+// is 2^13 units (128 KB of LDS) held by 512 threads, 16 units each, loaded and
+// stored with 16-byte accesses by consecutive lanes.  The LOW pass covers the
+// element bits [0, HB) (HB = 14 real, 13 complex); each high pass covers up
+// to 9 further qubits with rows of at least 16 units (256 B) in memory.
 template <typename T>
 struct HvCfg;
 template <>
 struct HvCfg<double> {
-    static constexpr int TB = 13;
+    static constexpr int HB = 14, EPU = 2;  // element bits of a tile, elements per unit
 };
 template <>
 struct HvCfg<double2> {
-    static constexpr int TB = 12;
+    static constexpr int HB = 13, EPU = 1;
 };
+constexpr int kHvUnitBits = 13, kHvThreadBits = 9, kHvRowUnitBits = 4;
+constexpr int kHvThreads = 1 << kHvThreadBits, kHvUPT = 1 << (kHvUnitBits - kHvThreadBits);
 
-// LOW pass: y = d*x - g * sum_{j<TB} x[s ^ 2^j] on contiguous tiles
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_hv_low(const T* __restrict__ x, T* __restrict__ y, uint64_t ntiles,
-                                                    int n, double g, double dscale)
+// registers and LDS use a native two-double vector: arrays of it are promoted
+// to registers, where arrays of the HIP vector struct stayed in scratch memory
+typedef double dv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ dv2 mkv(double a, double b)
 {
-    constexpr int TB = HvCfg<T>::TB;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lds = reinterpret_cast<T*>(smem_raw);
+    dv2 r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+__device__ __forceinline__ dv2 d2_axpy(double a, const dv2& x, const dv2& y)
+{
+    return mkv(fma(a, x.x, y.x), fma(a, x.y, y.y));
+}
+
+// LOW pass: y = d(E) x - g sum_{j<HB} x[s ^ 2^j] + (the high groups' sums in
+// acc0, acc1) on contiguous tiles, E(s) from the state-order uint16 cache
+template <typename T, int NACC>
+__global__ __launch_bounds__(kHvThreads) void k_hv_low(const T* __restrict__ x, T* y, uint64_t ntiles, double g,
+                                                     double dscale, const uint16_t* __restrict__ ecache,
+                                                     const T* acc0, const T* acc1)
+{
+    constexpr int EPU = HvCfg<T>::EPU;
+    __shared__ __align__(16) dv2 lds[1 << kHvUnitBits];
+    const uint32_t t = threadIdx.x;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t base = tile << TB;
-        __syncthreads();
-        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) lds[i] = x[base + i];
-        __syncthreads();
-        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) {
-            const uint32_t s = static_cast<uint32_t>(base + i);
-            T acc = scl(static_cast<double>(labs_energy_u32_l(s, n)) * dscale, lds[i]);
+        const uint64_t ubase = tile << kHvUnitBits;  // first unit of the tile
+        const dv2* xu = reinterpret_cast<const dv2*>(x) + ubase;
+        dv2 v[kHvUPT];
 #pragma unroll
-            for (int j = 0; j < TB; ++j) axpy_(acc, -g, lds[i ^ (1 << j)]);
-            y[base + i] = acc;
+        for (int k = 0; k < kHvUPT; ++k) v[k] = xu[(k << kHvThreadBits) + t];
+        __syncthreads();  // the previous tile's readers are done
+#pragma unroll
+        for (int k = 0; k < kHvUPT; ++k) lds[(k << kHvThreadBits) + t] = v[k];
+        __syncthreads();
+        // E(s) of the units' elements (two uint16 for real FP64) and the high
+        // groups' sums (written by the high passes), which start the
+        // accumulators: read once x's registers are free
+        uint32_t ew[kHvUPT];
+#pragma unroll
+        for (int k = 0; k < kHvUPT; ++k) {
+            const uint64_t gu = ubase + (k << kHvThreadBits) + t;
+            ew[k] = EPU == 2 ? reinterpret_cast<const uint32_t*>(ecache)[gu] : static_cast<uint32_t>(ecache[gu]);
+        }
+        dv2 ad[kHvUPT];
+#pragma unroll
+        for (int k = 0; k < kHvUPT; ++k) {
+            ad[k] = mkv(0.0, 0.0);
+            if constexpr (NACC >= 1) {
+                const dv2 w = reinterpret_cast<const dv2*>(acc0)[ubase + (k << kHvThreadBits) + t];
+                ad[k].x += w.x;
+                ad[k].y += w.y;
+            }
+            if constexpr (NACC >= 2) {
+                const dv2 w = reinterpret_cast<const dv2*>(acc1)[ubase + (k << kHvThreadBits) + t];
+                ad[k].x += w.x;
+                ad[k].y += w.y;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);  // these loads stay ahead of the unit loop
+        dv2* yu = reinterpret_cast<dv2*>(y) + ubase;
+        // the unit and its neighbours come from LDS, so x's registers are free
+        // for the accumulators.  Fully unrolled (register arrays indexed by a
+        // loop variable would live in scratch memory); the scheduling barrier
+        // keeps each unit's LDS reads after the previous unit's, so they are
+        // not all hoisted into registers at once.
+#pragma unroll
+        for (int k = 0; k < kHvUPT; ++k) {
+            __builtin_amdgcn_sched_barrier(0);
+            const uint32_t u = (k << kHvThreadBits) + t;
+            const dv2 self = lds[u];
+#pragma unroll
+            for (int j = 0; j < kHvUnitBits; ++j) ad[k] = d2_axpy(-g, lds[u ^ (1u << j)], ad[k]);
+            dv2 acc;
+            if constexpr (EPU == 2) {
+                // elements e0 (x) and e0 + 1 (y): bit 0 pairs them inside the unit
+                const uint32_t E0 = ew[k] & 0xFFFFu, E1 = ew[k] >> 16;
+                acc.x = fma(-g, self.y, fma(static_cast<double>(E0) * dscale, self.x, ad[k].x));
+                acc.y = fma(-g, self.x, fma(static_cast<double>(E1) * dscale, self.y, ad[k].y));
+            } else {
+                const uint32_t E = ew[k];
+                const double d = static_cast<double>(E) * dscale;
+                acc = mkv(fma(d, self.x, ad[k].x), fma(d, self.y, ad[k].y));
+            }
+            yu[u] = acc;
         }
     }
 }
 
-// HIGH pass: y += -g * sum_{j in [h, h+A)} x[s ^ 2^j]; tile = 2^R rows x 2^A
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_hv_high(const T* __restrict__ x, T* __restrict__ y, uint64_t ntiles,
-                                                     int R, int h, double g)
+// high pass: y = -g sum over the group's qubits of x[s ^ 2^q] (ACC: y += ...),
+// so that a pass reads one array where it can (ACC, used only past the
+// second group or without scratch, reads y as it goes).  The group is
+// a1 element bits at h1 and A - a1 at h2 (the sweeps' planner, which spreads
+// the page-crossing qubits over the groups); a tile is 2^(13-A) contiguous
+// units (a row) times the 2^A rows of the group.  The tile's x is read before
+// the LDS exchange.
+template <typename T, int A, bool ACC>
+__global__ __launch_bounds__(kHvThreads) void k_hv_high(const T* __restrict__ x, T* __restrict__ y, uint64_t ntiles,
+                                                      int h1, int a1, int h2, double g)
 {
-    constexpr int TB = HvCfg<T>::TB;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lds = reinterpret_cast<T*>(smem_raw);
-    const int A = TB - R;
-    const int lo_bits = h - R;
-    const uint32_t rowmask = (1u << R) - 1u;
+    constexpr int EPU = HvCfg<T>::EPU;
+    constexpr int RU = kHvUnitBits - A;  // unit bits of a row
+    constexpr uint32_t rowmask = (1u << RU) - 1u;
+    __shared__ __align__(16) dv2 lds[1 << kHvUnitBits];
+    const uint32_t t = threadIdx.x;
+    const int e = EPU == 2 ? 1 : 0;       // element bits inside a unit
+    const int hs1 = h1 - e, hs2 = h2 - e;  // the segments as unit bits
+    const int lo_bits = hs1 - RU, mid_bits = hs2 - (hs1 + a1);
+    const uint32_t m1 = (1u << a1) - 1u;
+    const dv2* xu = reinterpret_cast<const dv2*>(x);
+    dv2* yu = reinterpret_cast<dv2*>(y);
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t lo = tile & ((1ull << lo_bits) - 1ull);
-        const uint64_t hi = tile >> lo_bits;
-        const uint64_t base = (lo << R) | (hi << (h + A));
+        const uint64_t mid = (tile >> lo_bits) & ((1ull << mid_bits) - 1ull);
+        const uint64_t hi = tile >> (lo_bits + mid_bits);
+        const uint64_t ubase = (lo << RU) | (mid << (hs1 + a1)) | (hi << (hs2 + (A - a1)));
+        auto gu = [&](uint32_t u) -> uint64_t {
+            const uint32_t ua = u >> RU;
+            return ubase + (u & rowmask) + (static_cast<uint64_t>(ua & m1) << hs1) +
+                   (static_cast<uint64_t>(ua >> a1) << hs2);
+        };
+        dv2 v[kHvUPT];
+#pragma unroll
+        for (int k = 0; k < kHvUPT; ++k) v[k] = xu[gu((k << kHvThreadBits) + t)];
         __syncthreads();
-        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) {
-            const uint64_t gi = base + (i & rowmask) + (static_cast<uint64_t>(i >> R) << h);
-            lds[i] = x[gi];
-        }
+#pragma unroll
+        for (int k = 0; k < kHvUPT; ++k) lds[(k << kHvThreadBits) + t] = v[k];
         __syncthreads();
-        for (int i = threadIdx.x; i < (1 << TB); i += kThreads) {
-            const uint64_t gi = base + (i & rowmask) + (static_cast<uint64_t>(i >> R) << h);
-            T acc = y[gi];
-            for (int j = 0; j < A; ++j) axpy_(acc, -g, lds[i ^ (1 << (R + j))]);
-            y[gi] = acc;
+        // fully unrolled (register arrays indexed by a loop variable would live
+        // in scratch memory); the scheduling barrier keeps each unit's LDS
+        // reads after the previous unit's, so they are not all hoisted at once
+#pragma unroll
+        for (int k = 0; k < kHvUPT; ++k) {
+            __builtin_amdgcn_sched_barrier(0);
+            const uint32_t u = (k << kHvThreadBits) + t;
+            dv2 sm = mkv(0.0, 0.0);
+#pragma unroll
+            for (int b = RU; b < kHvUnitBits; ++b) {
+                const dv2 w = lds[u ^ (1u << b)];
+                sm.x += w.x;
+                sm.y += w.y;
+            }
+            const uint64_t gk = gu(u);
+            yu[gk] = ACC ? d2_axpy(-g, sm, yu[gk]) : mkv(-g * sm.x, -g * sm.y);
         }
+    }
+}
+
+template <typename T, int A = 1>
+hipError_t launch_hv_high(int a, bool acc, const T* x, T* y, uint64_t ntiles, int h1, int a1, int h2, double g,
+                          int grid, hipStream_t st)
+{
+    if constexpr (A > kHvUnitBits - kHvRowUnitBits) {
+        return hipErrorInvalidValue;
+    } else {
+        if (a != A) return launch_hv_high<T, A + 1>(a, acc, x, y, ntiles, h1, a1, h2, g, grid, st);
+        if (acc)
+            hipLaunchKernelGGL((k_hv_high<T, A, true>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, h1, a1, h2,
+                               g);
+        else
+            hipLaunchKernelGGL((k_hv_high<T, A, false>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, h1, a1,
+                               h2, g);
+        return hipGetLastError();
     }
 }
 
@@ -286,46 +407,102 @@ __global__ void k_hv_rows(const T* __restrict__ x, T* __restrict__ y, int n, dou
 }
 
 template <typename T>
-static hipError_t hv_impl(int n, double g, double dscale, const T* x, T* y, hipStream_t st)
+static hipError_t launch_hv_low(int nacc, const T* x, T* y, uint64_t ntiles, double g, double dscale,
+                                const uint16_t* ecache, const T* acc0, const T* acc1, int grid, hipStream_t st)
 {
-    constexpr int TB = HvCfg<T>::TB;
+    switch (nacc) {
+    case 0:
+        hipLaunchKernelGGL((k_hv_low<T, 0>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, g, dscale, ecache,
+                           acc0, acc1);
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_hv_low<T, 1>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, g, dscale, ecache,
+                           acc0, acc1);
+        break;
+    default:
+        hipLaunchKernelGGL((k_hv_low<T, 2>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, g, dscale, ecache,
+                           acc0, acc1);
+        break;
+    }
+    return hipGetLastError();
+}
+
+// the high-qubit groups of H.v on n spins (qubits [HB, n) in the fewest groups
+// of at most 9, rows of at least 16 units, the page-crossing qubits of 8 MB
+// pages spread over the groups)
+template <typename T>
+static std::vector<HighSweep> hv_groups(int n)
+{
+    constexpr int EB = HvCfg<T>::EPU == 2 ? 1 : 0;
+    constexpr int page_bits = HvCfg<T>::EPU == 2 ? 20 : 19;
+    return plan_high_sweeps(n, HvCfg<T>::HB, HvCfg<T>::HB, kHvRowUnitBits + EB, page_bits);
+}
+
+// High passes first, each writing its group's sum: the first into y, the
+// second into `scratch` (one vector; null: it and any further group add into
+// y), further groups adding into the scratch vector; then the LOW pass adds
+// d(E) x and its own qubits' terms to them and writes y.
+template <typename T>
+static hipError_t hv_impl(int n, double g, double dscale, const T* x, T* y, const uint16_t* ecache, T* scratch,
+                          hipStream_t st)
+{
+    constexpr int HB = HvCfg<T>::HB;
     const uint64_t dim = 1ull << n;
     hipError_t e;
-    if (n < TB) {
+    if (n < HB) {
         hipLaunchKernelGGL(k_hv_rows<T>, dim3(static_cast<unsigned>((dim + 255) / 256)), dim3(256), 0, st, x, y, n,
                            g, dscale);
         return hipGetLastError();
     }
-    const size_t lds = sizeof(T) << TB;
-    const uint64_t ntiles = dim >> TB;
-    const int grid = static_cast<int>(std::min<uint64_t>(ntiles, 2048));
-    hipLaunchKernelGGL(k_hv_low<T>, dim3(grid), dim3(kThreads), lds, st, x, y, ntiles, n, g, dscale);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int rmin = 4;
-    const int amax = TB - rmin;
-    const int remaining = n - TB;
-    if (remaining <= 0) return hipSuccess;
-    const int nh = (remaining + amax - 1) / amax;
-    int h = TB;
-    for (int i = 0; i < nh; ++i) {
-        const int left = n - h;
-        const int A = (left + (nh - i) - 1) / (nh - i);
-        const int R = TB - A;
-        hipLaunchKernelGGL(k_hv_high<T>, dim3(grid), dim3(kThreads), lds, st, x, y, ntiles, R, h, g);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        h += A;
+    if (!ecache) return hipErrorInvalidValue;  // the tiled passes read E(s) from the cache
+    const uint64_t ntiles = dim >> HB;
+    const int grid = static_cast<int>(std::min<uint64_t>(ntiles, 4096));
+    const std::vector<HighSweep> groups = hv_groups<T>(n);
+    for (size_t i = 0; i < groups.size(); ++i) {
+        const HighSweep& hs = groups[i];
+        T* out = (i >= 1 && scratch) ? scratch : y;
+        const bool acc = scratch ? i >= 2 : i >= 1;
+        if ((e = launch_hv_high<T>(hs.a1 + hs.a2, acc, x, out, ntiles, hs.h1, hs.a1, hs.h2, g, grid, st)) !=
+            hipSuccess)
+            return e;
     }
-    return hipSuccess;
+    const int nacc = groups.empty() ? 0 : (scratch && groups.size() >= 2 ? 2 : 1);
+    return launch_hv_low<T>(nacc, x, y, ntiles, g, dscale, ecache, y, scratch, grid, st);
+}
+
+size_t hv_scratch_doubles(int n, int is_complex)
+{
+    if (n < 1 || n > 32) return 0;
+    const size_t groups = is_complex ? hv_groups<double2>(n).size() : hv_groups<double>(n).size();
+    if (groups < 2) return 0;
+    return (static_cast<size_t>(1) << n) * (is_complex ? 2 : 1);
 }
 
 hipError_t launch_hv(int n, double g, int normalize_by_n, int is_complex, const double* x, double* y,
-                     hipStream_t stream)
+                     hipStream_t stream, const uint16_t* ecache, double* scratch)
 {
     if (n < 1 || n > 32) return hipErrorInvalidValue;
     const double dscale = normalize_by_n ? 1.0 / static_cast<double>(n) : 1.0;
     if (is_complex)
-        return hv_impl(n, g, dscale, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), stream);
-    return hv_impl(n, g, dscale, x, y, stream);
+        return hv_impl(n, g, dscale, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), ecache,
+                       reinterpret_cast<double2*>(scratch), stream);
+    return hv_impl(n, g, dscale, x, y, ecache, scratch, stream);
+}
+
+__global__ __launch_bounds__(kThreads) void k_energy_natural(uint16_t* __restrict__ out, int n, uint64_t dim)
+{
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        out[i] = static_cast<uint16_t>(labs_energy_u32_l(static_cast<uint32_t>(i), n));
+}
+
+hipError_t launch_energy_natural(uint16_t* out, int n, hipStream_t stream)
+{
+    if (n < 1 || n > 32) return hipErrorInvalidValue;
+    const uint64_t dim = 1ull << n;
+    const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 65536));
+    hipLaunchKernelGGL(k_energy_natural, dim3(grid), dim3(kThreads), 0, stream, out, n, dim);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -469,6 +646,99 @@ hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V, c
 {
     const int grid = static_cast<int>(std::min<int64_t>((len + 255) / 256, 8192));
     hipLaunchKernelGGL(k_combine, dim3(std::max(grid, 1)), dim3(256), 0, st, len, is_complex, m, V, coef, y);
+    return hipGetLastError();
+}
+
+// ---- Gram-Schmidt against a block of basis vectors (complex128) ----
+// k_multidot: per workgroup partial sums of c_i = <V_i, w> for i < m <= kMdMax,
+// one pass over w and the m vectors; k_multidot_finish sums them over the
+// workgroups in a fixed order (coef[2i], coef[2i+1]).  k_gs_sub: w -= sum_i
+// c_i V_i with the coefficients read from device memory.
+constexpr int kMdGrid = 1024;
+
+__global__ __launch_bounds__(kThreads) void k_multidot(int64_t len, int m, const double* __restrict__ V,
+                                                      const double* __restrict__ w, int64_t chunk,
+                                                      double* __restrict__ partials)
+{
+    __shared__ double sc[kThreads / 64][2 * kMdMax];
+    double ar[kMdMax], ai[kMdMax];
+#pragma unroll
+    for (int i = 0; i < kMdMax; ++i) ar[i] = ai[i] = 0.0;
+    const int64_t beg = blockIdx.x * chunk;
+    const int64_t end = min(len, beg + chunk);
+    for (int64_t e = beg + threadIdx.x; e < end; e += kThreads) {
+        const double2 wv = reinterpret_cast<const double2*>(w)[e];
+#pragma unroll
+        for (int i = 0; i < kMdMax; ++i) {
+            if (i < m) {
+                const double2 v = reinterpret_cast<const double2*>(V + 2 * len * i)[e];
+                ar[i] = fma(v.x, wv.x, fma(v.y, wv.y, ar[i]));
+                ai[i] = fma(v.x, wv.y, fma(-v.y, wv.x, ai[i]));
+            }
+        }
+    }
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kMdMax; ++i) {
+        if (i < m) {
+            const double r = wsum(ar[i]), q = wsum(ai[i]);
+            if ((threadIdx.x & 63) == 0) {
+                sc[wave][2 * i] = r;
+                sc[wave][2 * i + 1] = q;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * m) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < kThreads / 64; ++k) a += sc[k][threadIdx.x];
+        partials[static_cast<int64_t>(blockIdx.x) * 2 * kMdMax + threadIdx.x] = a;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_multidot_finish(const double* __restrict__ partials, int blocks, int m,
+                                                             double* __restrict__ coef)
+{
+    if (threadIdx.x < 2 * m) {
+        double a = 0.0;
+        for (int b = 0; b < blocks; ++b) a += partials[static_cast<int64_t>(b) * 2 * kMdMax + threadIdx.x];
+        coef[threadIdx.x] = a;
+    }
+}
+
+__global__ void k_gs_sub(int64_t len, int m, const double* __restrict__ V, const double* __restrict__ coef,
+                         double* __restrict__ w)
+{
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < len;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double2 acc = reinterpret_cast<const double2*>(w)[e];
+        for (int i = 0; i < m; ++i) {
+            const double cr = coef[2 * i], ci = coef[2 * i + 1];
+            const double2 v = reinterpret_cast<const double2*>(V + 2 * len * i)[e];
+            acc.x = fma(-cr, v.x, fma(ci, v.y, acc.x));
+            acc.y = fma(-cr, v.y, fma(-ci, v.x, acc.y));
+        }
+        reinterpret_cast<double2*>(w)[e] = acc;
+    }
+}
+
+size_t multidot_partials_doubles() { return static_cast<size_t>(kMdGrid) * 2 * kMdMax; }
+
+hipError_t launch_gs_block(int64_t len, int m, const double* V, double* w, double* coef, double* partials,
+                           hipStream_t st)
+{
+    if (m < 1 || m > kMdMax) return hipErrorInvalidValue;
+    int64_t chunk = 4096;
+    while ((len + chunk - 1) / chunk > kMdGrid) chunk <<= 1;
+    const int grid = static_cast<int>(std::max<int64_t>(1, (len + chunk - 1) / chunk));
+    hipLaunchKernelGGL(k_multidot, dim3(grid), dim3(kThreads), 0, st, len, m, V, w, chunk, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_multidot_finish, dim3(1), dim3(kThreads), 0, st, partials, grid, m, coef);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int sgrid = static_cast<int>(std::min<int64_t>((len + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_gs_sub, dim3(std::max(sgrid, 1)), dim3(256), 0, st, len, m, V, coef, w);
     return hipGetLastError();
 }
 
