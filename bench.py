@@ -131,6 +131,58 @@ def _adam_bench(bk, eng, seed: int, iters: int = 50, lr: float = 0.005) -> dict:
             "g_min": float(res["g"].min()), "g_max": float(res["g"].max())}
 
 
+def _linalg_bench(bk, eng, seed: int) -> dict:
+    """The subspace side and sampling: the matrix-free H.v (real FP64 at N=30,
+    complex128 at N=28; device tensors, five timed calls after a warm one), one
+    Krylov step of the propagator at N=26 (m=30, complex128), and 100 projective
+    samples of the evolved N=30 complex64 state.  H.v rates count the bytes its
+    passes move: each high-qubit group reads x and writes its sum (2 x 8 B per
+    real element), the LOW pass reads x, E(s) (2 B) and the groups' sums and
+    writes y."""
+    import torch
+    out = {}
+
+    def timed(f, reps=5):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    for n, cplx in ((30, False), (28, True)):
+        dt = torch.complex128 if cplx else torch.float64
+        x = torch.randn(1 << n, dtype=dt, device="cuda")
+        y = torch.empty_like(x)
+        sec = timed(lambda: eng.hv(n, 0.35, x, y))
+        b = 16 if cplx else 8                                   # bytes per element
+        hb = 13 if cplx else 14                                 # element bits of a tile
+        ng = max(0, -(-(n - hb) // 9))                          # high groups of at most 9 qubits
+        # LOW: x read, y written, E(s), up to two group sums read; each group: x read, sum written
+        moved = (1 << n) * ((2 + min(ng, 2)) * b + 2 + ng * 2 * b)
+        out[f"hv_n{n}_{'c128' if cplx else 'f64'}"] = {"ms": sec * 1e3, "high_groups": ng,
+                                                       "bytes_moved": moved, "gbs_moved": moved / sec / 1e9}
+        del x, y
+        torch.cuda.empty_cache()
+    n = 26
+    _, s0, _ = bk.make_trap(n, seed)
+    eng.state_init(n, bk.BK_C128, s0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.krylov_propagate(0.35, 0.25, tau_restart=0.25, m=30)
+    torch.cuda.synchronize()
+    out["krylov_n26_m30_restart_s"] = time.perf_counter() - t0
+    n = 30
+    _, s0, e0 = bk.make_trap(n, seed)
+    eng.evolve(n, s0, 0.35, 0.02, 10, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0, measure_every=0)
+    import numpy as np
+    u = np.random.default_rng(seed).random(100)
+    sec = timed(lambda: eng.sample(u))
+    out["sample_n30_c64_100_shots"] = {"ms": sec * 1e3, "gbs_one_read": 8.0 * (1 << n) / sec / 1e9}
+    return out
+
+
 def _batch_bench(bk, eng, seed: int) -> dict:
     """Batched workload: 512 traps, n drawn from {16, 18, 20, 22} (mt19937_64),
     16 fields evenly spaced in [0.10, 0.60], Strang dt=0.02 x 500 steps, c64,
@@ -238,6 +290,8 @@ def main() -> int:
                     help="skip the 512-trap batched timing")
     ap.add_argument("--no-traps", action="store_true",
                     help="skip the trap-supply (batched steepest descent) timing")
+    ap.add_argument("--no-linalg", action="store_true",
+                    help="skip the H.v / Krylov / sampling timings")
     ap.add_argument("--adam", action="store_true",
                     help="also run 50 Adam steps on the N=26 adjoint workload's fields")
     ap.add_argument("--no-fp64-check", action="store_true",
@@ -328,6 +382,11 @@ def main() -> int:
         print("[bench] adjoint N=26", file=sys.stderr, flush=True)
         adj = _adjoint_bench(bk, eng, args.seed)
 
+    linalg = None
+    if not args.no_linalg:
+        print("[bench] H.v / Krylov / sampling", file=sys.stderr, flush=True)
+        linalg = _linalg_bench(bk, eng, args.seed)
+
     adam = None
     if args.adam:
         print("[bench] 50 Adam steps N=26", file=sys.stderr, flush=True)
@@ -398,6 +457,7 @@ def main() -> int:
         },
         "adjoint": adj,
         "adam": adam,
+        "linalg": linalg,
         "batched": batch,
         "trap_supply": traps,
         "cpu_baseline": cpu,
