@@ -183,6 +183,32 @@ def _linalg_bench(bk, eng, seed: int) -> dict:
     return out
 
 
+# quantum_time_sec of the reference's own run of this configuration and trap
+# (data_processed/probability_dynamics_n30_results.csv:2, N=30 g=0.3 trap 0x38C06C95)
+REF_SAMPLER_N30_S = 324.6727
+
+
+def _sampler_e2e_bench(bk) -> dict:
+    """ISampler::sample end to end on the reference's N=30 probability-dynamics
+    configuration (configs/probability_scan_n30.cfg: Lie dt=0.05 to t=60, 1200
+    steps, P_< and <d_H> recorded every step, no shots) for g=0.3 and the
+    reference's trap 0x38C06C95, in complex128 (the reference's precision) and
+    complex64; wall time beside the reference's recorded time for the run."""
+    config = importlib.import_module("bernasconi-krylov_amd.config")
+    cfg = config.load_from_file(os.path.join(ROOT, "tests", "golden", "configs", "probability_scan_n30.cfg"))
+    out = {"reference_s": REF_SAMPLER_N30_S}
+    for name, prec in (("c128", bk.BK_C128), ("c64", bk.BK_C64)):
+        sc = config.to_sampler_config(cfg.quantum, 30, 0.3, 42)
+        sc.precision = prec
+        sc.verbose = False
+        t0 = time.perf_counter()
+        r = bk.create_sampler(30).sample(0x38C06C95, sc)
+        sec = time.perf_counter() - t0
+        out[name] = {"wall_s": sec, "speedup": REF_SAMPLER_N30_S / sec, "peak_p_less": r.peak_p_less,
+                     "peak_time": r.peak_time, "p_blind": r.p_blind, "points": len(r.time_points)}
+    return out
+
+
 def _batch_bench(bk, eng, seed: int) -> dict:
     """Batched workload: 512 traps, n drawn from {16, 18, 20, 22} (mt19937_64),
     16 fields evenly spaced in [0.10, 0.60], Strang dt=0.02 x 500 steps, c64,
@@ -292,6 +318,8 @@ def main() -> int:
                     help="skip the trap-supply (batched steepest descent) timing")
     ap.add_argument("--no-linalg", action="store_true",
                     help="skip the H.v / Krylov / sampling timings")
+    ap.add_argument("--sampler-e2e", action="store_true",
+                    help="also time ISampler::sample on the reference's N=30 configuration and trap")
     ap.add_argument("--adam", action="store_true",
                     help="also run 50 Adam steps on the N=26 adjoint workload's fields")
     ap.add_argument("--no-fp64-check", action="store_true",
@@ -387,6 +415,11 @@ def main() -> int:
         print("[bench] H.v / Krylov / sampling", file=sys.stderr, flush=True)
         linalg = _linalg_bench(bk, eng, args.seed)
 
+    e2e = None
+    if args.sampler_e2e:
+        print("[bench] sampler run, N=30 reference configuration", file=sys.stderr, flush=True)
+        e2e = _sampler_e2e_bench(bk)
+
     adam = None
     if args.adam:
         print("[bench] 50 Adam steps N=26", file=sys.stderr, flush=True)
@@ -457,6 +490,7 @@ def main() -> int:
         },
         "adjoint": adj,
         "adam": adam,
+        "sampler_e2e": e2e,
         "linalg": linalg,
         "batched": batch,
         "trap_supply": traps,
