@@ -344,12 +344,17 @@ def main() -> int:
         rows = {}
         for mb in (8, 16, 32, 64, 128, 256, 1024, 8192):  # bytes copied each way; working set is twice that
             rows[f"{2 * mb} MB working set"] = eng.copy_probe(mb << 20, 50 if mb <= 256 else 5)
+            rows[f"{2 * mb} MB working set, 4 KB workgroups"] = eng.copy_probe(mb << 20, 50 if mb <= 256 else 5,
+                                                                             pattern=1)
         print(json.dumps({"copy_gbs_read_plus_write": rows}))
         eng.close()
         return 0
     if args.copy_probe:
         eng = bk.Engine(0)
-        print(json.dumps({"copy_probe_bytes": 1 << 33, "gbs_read_plus_write": eng.copy_probe(1 << 33, 5)}))
+        print(json.dumps({"copy_probe_bytes": 1 << 33,
+                          "gbs_read_plus_write": {"grid_stride": eng.copy_probe(1 << 33, 5),
+                                                  "4KB_workgroups": eng.copy_probe(1 << 33, 5, pattern=1),
+                                                  "128KB_workgroups": eng.copy_probe(1 << 33, 5, pattern=2)}}))
         eng.close()
         return 0
     prec = bk.BK_C64 if args.precision == "c64" else bk.BK_C128
@@ -403,7 +408,12 @@ def main() -> int:
                 "dh_max_abs_err": float(np.abs(np.asarray(res["dh"]) - rd).max()),
                 "norm_max_abs_dev": float(np.abs(np.asarray(res["norm"]) - 1.0).max())}
 
-    copy_gbs = eng.copy_probe(1 << 33, 3)  # practical read+write ceiling on this device
+    # read+write copies of 8 GiB on this device: the grid-stride copy, the part's
+    # fastest stream (4 KB per workgroup) and a copy whose workgroups hold 128 KB,
+    # as a complex64 sweep tile does
+    copy_gbs = eng.copy_probe(1 << 33, 3)
+    copy_best_gbs = eng.copy_probe(1 << 33, 3, pattern=1)
+    copy_tile_gbs = eng.copy_probe(1 << 33, 3, pattern=2)
 
     adj = None
     if not args.no_adjoint:
@@ -487,6 +497,10 @@ def main() -> int:
             "traffic": _traffic(dom["name"], n, args.precision, args.energy),
             "copy_ceiling_gbs": copy_gbs,
             "frac_of_copy_ceiling": achieved / copy_gbs,
+            "copy_best_gbs": copy_best_gbs,
+            "frac_of_copy_best": achieved / copy_best_gbs,
+            "copy_tile_gbs": copy_tile_gbs,
+            "frac_of_copy_tile": achieved / copy_tile_gbs,
         },
         "adjoint": adj,
         "adam": adam,

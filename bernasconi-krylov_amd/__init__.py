@@ -181,6 +181,7 @@ _SIGS = {
     "bk_sweep_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     "bk_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "bk_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "bk_copy_probe_pattern": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "bk_subspace_lanczos": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_uint64), C.c_int64,
                                       C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bk_subspace_dense": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_uint64), C.c_int64,
@@ -510,10 +511,15 @@ class Engine:
                  "total_ms": buf[i].total_ms, "bytes_per_launch": buf[i].bytes_per_launch}
                 for i in range(cnt.value)]
 
-    def copy_probe(self, nbytes: int = 1 << 33, reps: int = 5) -> float:
-        """GB/s (read+write) of a plain device copy of nbytes."""
+    def copy_probe(self, nbytes: int = 1 << 33, reps: int = 5, pattern: int = 0) -> float:
+        """GB/s (read+write) of a device copy of nbytes.  pattern 0: grid-stride
+        copy; 1: 4 KB per workgroup, nontemporal (the part's fastest stream);
+        2: 128 KB per workgroup (a complex64 sweep tile's footprint)."""
         g = C.c_double()
-        _check(lib().bk_copy_probe(self._h, nbytes, reps, C.byref(g)))
+        if pattern == 0:
+            _check(lib().bk_copy_probe(self._h, nbytes, reps, C.byref(g)))
+        else:
+            _check(lib().bk_copy_probe_pattern(self._h, nbytes, reps, pattern, C.byref(g)))
         return g.value
 
     # -- subspace

@@ -1020,7 +1020,7 @@ static int bk_krylov_propagate_impl(bk_ctx* c, double g, int normalize_by_n, dou
     HIPCALL(V.ensure(sizeof(double) * 2 * static_cast<size_t>(dim) * m));
     HIPCALL(w.ensure(sizeof(double) * 2 * static_cast<size_t>(dim)));
     HIPCALL(coef.ensure(sizeof(double) * 2 * m));
-    HIPCALL(gsc.ensure(sizeof(double) * 4 * static_cast<size_t>(bk::kMdMax)));  // coefficients of both sweeps
+    HIPCALL(gsc.ensure(sizeof(double) * (4 * static_cast<size_t>(bk::kMdMax) + 2)));  // both sweeps' coefficients, |w|^2
     HIPCALL(gsp.ensure(sizeof(double) * bk::multidot_partials_doubles()));
     HIPCALL(c->blas_partials.ensure(sizeof(double) * bk::blas_partials_doubles()));
     HIPCALL(c->blas_out.ensure(sizeof(double) * 4));
@@ -1050,24 +1050,40 @@ static int bk_krylov_propagate_impl(bk_ctx* c, double g, int normalize_by_n, dou
         for (int j = 0; j < m; ++j) {
             double* vj = dV + static_cast<size_t>(j) * 2 * dim;
             HIPCALL(bk::launch_hv(n, g, normalize_by_n, 1, vj, dw, c->stream, ec, scr));
-            // classical Gram-Schmidt twice against v_0..v_j, in blocks of at most
-            // kMdMax vectors; the coefficient of v_j in each sweep adds to alpha_j
-            // (each sweep keeps its coefficients in its own region; the last
-            // block of a sweep, which holds v_j, is the one left there)
+            // classical Gram-Schmidt twice against v_0..v_j; the coefficient of
+            // v_j in each sweep adds to alpha_j (each sweep keeps its coefficients
+            // in its own region of dgs)
             double* dgs = gsc.as<double>();
-            for (int pass = 0; pass < 2; ++pass)
-                for (int i0 = 0; i0 <= j; i0 += bk::kMdMax) {
-                    const int mm = std::min(bk::kMdMax, j + 1 - i0);
-                    HIPCALL(bk::launch_gs_block(dim, mm, dV + static_cast<size_t>(i0) * 2 * dim, dw,
-                                                dgs + 2 * bk::kMdMax * pass, gsp.as<double>(), c->stream));
-                }
-            const int last = j % bk::kMdMax;  // v_j's position in its block
+            double* vn = j + 1 < m ? vj + 2 * dim : nullptr;
+            const bool fused = j + 1 <= bk::kMdMax;
             double aj[2] = {0.0, 0.0};
-            for (int pass = 0; pass < 2; ++pass)
-                HIPCALL(hipMemcpyAsync(aj + pass, dgs + 2 * bk::kMdMax * pass + 2 * last, sizeof(double),
-                                       hipMemcpyDeviceToHost, c->stream));
             double bsq = 0.0;
-            if ((rc = zdot(dw, dw, &bsq, nullptr))) return rc;  // synchronises the stream (aj too)
+            if (fused) {
+                // one block: three passes, the second sweep's subtraction writing
+                // v_{j+1} (unnormalised) and |v_{j+1}|^2
+                HIPCALL(bk::launch_cgs2(dim, j + 1, dV, dw, vn, dgs, gsp.as<double>(), c->stream));
+                for (int pass = 0; pass < 2; ++pass)
+                    HIPCALL(hipMemcpyAsync(aj + pass, dgs + 2 * bk::kMdMax * pass + 2 * j, sizeof(double),
+                                           hipMemcpyDeviceToHost, c->stream));
+                if (vn)
+                    HIPCALL(hipMemcpyAsync(&bsq, dgs + 4 * bk::kMdMax, sizeof(double), hipMemcpyDeviceToHost,
+                                           c->stream));
+                HIPCALL(hipStreamSynchronize(c->stream));
+            } else {
+                // blocks of at most kMdMax vectors; the last block of a sweep,
+                // which holds v_j, is the one left in its region
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int i0 = 0; i0 <= j; i0 += bk::kMdMax) {
+                        const int mb = std::min(bk::kMdMax, j + 1 - i0);
+                        HIPCALL(bk::launch_gs_block(dim, mb, dV + static_cast<size_t>(i0) * 2 * dim, dw,
+                                                    dgs + 2 * bk::kMdMax * pass, gsp.as<double>(), c->stream));
+                    }
+                const int last = j % bk::kMdMax;  // v_j's position in its block
+                for (int pass = 0; pass < 2; ++pass)
+                    HIPCALL(hipMemcpyAsync(aj + pass, dgs + 2 * bk::kMdMax * pass + 2 * last, sizeof(double),
+                                           hipMemcpyDeviceToHost, c->stream));
+                if ((rc = zdot(dw, dw, &bsq, nullptr))) return rc;  // synchronises the stream (aj too)
+            }
             al[j] = aj[0] + aj[1];
             if (j == m - 1) break;
             const double bj = std::sqrt(bsq);
@@ -1076,8 +1092,8 @@ static int bk_krylov_propagate_impl(bk_ctx* c, double g, int normalize_by_n, dou
                 break;
             }
             be[j] = bj;
-            double* vn = vj + 2 * dim;
-            HIPCALL(hipMemcpyAsync(vn, dw, sizeof(double) * 2 * dim, hipMemcpyDeviceToDevice, c->stream));
+            if (!fused)
+                HIPCALL(hipMemcpyAsync(vn, dw, sizeof(double) * 2 * dim, hipMemcpyDeviceToDevice, c->stream));
             HIPCALL(bk::launch_scal(2 * dim, 1.0 / bj, vn, c->stream));
         }
         std::vector<double> d(al.begin(), al.begin() + mm);
@@ -1446,21 +1462,25 @@ int bk_sweep_plan(int n, int precision, int row_bits, int page_bits, int* out, i
     return bkc::guard_abi([&] { return bk_sweep_plan_impl(n, precision, row_bits, page_bits, out, cap); });
 }
 
-static int bk_copy_probe_impl(bk_ctx* c, uint64_t bytes, int reps, double* gbs)
+static int bk_copy_probe_impl(bk_ctx* c, uint64_t bytes, int reps, int pattern, double* gbs)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
-    if (bytes < 16 || bytes % 16 || reps < 1 || !gbs) return fail(BK_E_INVALID, "bk_copy_probe: bad argument");
+    if (pattern < 0 || pattern > 2) return fail(BK_E_INVALID, "bk_copy_probe: pattern %d", pattern);
+    const uint64_t chunk = bk::copy_pattern_chunk(pattern);
+    if (bytes < chunk || bytes % chunk || reps < 1 || !gbs)
+        return fail(BK_E_INVALID, "bk_copy_probe: bad argument (bytes must be a positive multiple of %llu)",
+                    static_cast<unsigned long long>(chunk));
     ScopedBuf a, b;
     HIPCALL(a.ensure(bytes));
     HIPCALL(b.ensure(bytes));
     HIPCALL(hipMemsetAsync(a.p, 0, bytes, c->stream));
-    HIPCALL(bk::launch_copy_probe(a.p, b.p, bytes, c->stream));  // warm
+    HIPCALL(bk::launch_copy_pattern(pattern, a.p, b.p, bytes, c->stream));  // warm
     ScopedEvent e0, e1;
     HIPCALL(e0.create());
     HIPCALL(e1.create());
     HIPCALL(hipEventRecord(e0.e, c->stream));
-    for (int r = 0; r < reps; ++r) HIPCALL(bk::launch_copy_probe(a.p, b.p, bytes, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCALL(bk::launch_copy_pattern(pattern, a.p, b.p, bytes, c->stream));
     HIPCALL(hipEventRecord(e1.e, c->stream));
     HIPCALL(hipEventSynchronize(e1.e));
     float ms = 0.0f;
@@ -1471,7 +1491,12 @@ static int bk_copy_probe_impl(bk_ctx* c, uint64_t bytes, int reps, double* gbs)
 
 int bk_copy_probe(bk_ctx* c, uint64_t bytes, int reps, double* gbs)
 {
-    return bkc::guard_abi([&] { return bk_copy_probe_impl(c, bytes, reps, gbs); });
+    return bkc::guard_abi([&] { return bk_copy_probe_impl(c, bytes, reps, 0, gbs); });
+}
+
+int bk_copy_probe_pattern(bk_ctx* c, uint64_t bytes, int reps, int pattern, double* gbs)
+{
+    return bkc::guard_abi([&] { return bk_copy_probe_impl(c, bytes, reps, pattern, gbs); });
 }
 
 static int bk_timing_enable_impl(bk_ctx* c, int enable)

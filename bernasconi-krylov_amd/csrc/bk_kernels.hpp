@@ -274,6 +274,14 @@ constexpr int kMdMax = 32;
 hipError_t launch_gs_block(int64_t len, int m, const double* V, double* w, double* coef, double* partials,
                            hipStream_t stream);
 size_t multidot_partials_doubles();
+// Both sweeps of classical Gram-Schmidt (CGS2) of complex128 w against the
+// m <= kMdMax vectors of V in three passes: c1 = V^H w; w -= V c1 with
+// c2 = V^H w on the new w; next = w - V c2 with |next|^2.  coef: 4 kMdMax + 2
+// doubles, c1 at [0, 2m), c2 at [2 kMdMax, 2 kMdMax + 2m), |next|^2 at
+// [4 kMdMax].  next = nullptr stops after c2 (w then holds the first sweep's
+// result).  next must not alias w or V's m vectors.
+hipError_t launch_cgs2(int64_t len, int m, const double* V, double* w, double* next, double* coef, double* partials,
+                       hipStream_t stream);
 
 // ---- subspace Hamiltonian over a sorted basis (diagonalizer.cuh:79-114, 188-245) ----
 // nbr[i * n + j] = index of basis[i] ^ 2^j in the sorted basis, or -1
@@ -292,5 +300,10 @@ hipError_t launch_descent(int n, int64_t count, const uint64_t* starts, uint64_t
 
 // plain float4 copy of 'bytes' (multiple of 16): HBM ceiling probe
 hipError_t launch_copy_probe(const void* src, void* dst, uint64_t bytes, hipStream_t stream);
+// copy of 'bytes' in access pattern 0 (the grid-stride copy above), 1 (4 KB
+// per workgroup, one-shot grid, nontemporal) or 2 (128 KB per workgroup, the
+// footprint of a complex64 sweep tile); bytes a multiple of the pattern's chunk
+hipError_t launch_copy_pattern(int pattern, const void* src, void* dst, uint64_t bytes, hipStream_t stream);
+uint64_t copy_pattern_chunk(int pattern);
 
 }  // namespace bk

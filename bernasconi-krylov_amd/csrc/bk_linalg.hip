@@ -650,77 +650,130 @@ hipError_t launch_combine(int64_t len, int is_complex, int m, const double* V, c
 }
 
 // ---- Gram-Schmidt against a block of basis vectors (complex128) ----
-// k_multidot: per workgroup partial sums of c_i = <V_i, w> for i < m <= kMdMax,
-// one pass over w and the m vectors; k_multidot_finish sums them over the
-// workgroups in a fixed order (coef[2i], coef[2i+1]).  k_gs_sub: w -= sum_i
-// c_i V_i with the coefficients read from device memory.
-constexpr int kMdGrid = 1024;
+// One kernel, k_gs<M, MODE>, for the sweeps of classical Gram-Schmidt against
+// M <= kMdMax basis vectors (a compile-time count, so the M loads of an
+// element issue back to back):
+//   GS_DOT       partial sums of c_i = <V_i, w>                  (reads V, w)
+//   GS_SUB       w -= sum_i c_i V_i                              (reads V, w; writes w)
+//   GS_SUB_DOT   w -= sum_i c_i V_i, then partial sums of <V_i, w> on the new w
+//                (the second sweep's dots ride on the first sweep's subtraction)
+//   GS_SUB_NORM  out = w - sum_i c_i V_i, partial sum of |out|^2 (reads V, w; writes out)
+// A workgroup reduces a contiguous chunk; k_gs_finish sums the workgroups'
+// partials in a fixed order, so every coefficient is deterministic.
+constexpr int kMdGrid = 16384;  // most workgroups of a sweep (partials: kMdGrid x 2 kMdMax doubles)
+enum GsMode { GS_DOT = 0, GS_SUB = 1, GS_SUB_DOT = 2, GS_SUB_NORM = 3 };
 
-__global__ __launch_bounds__(kThreads) void k_multidot(int64_t len, int m, const double* __restrict__ V,
-                                                      const double* __restrict__ w, int64_t chunk,
-                                                      double* __restrict__ partials)
+template <int M, int MODE>
+__global__ __launch_bounds__(kThreads) void k_gs(int64_t len, const double* __restrict__ V, double* w,
+                                                 double* __restrict__ out, const double* __restrict__ coef,
+                                                 int64_t chunk, double* __restrict__ partials)
 {
+    constexpr bool SUB = MODE != GS_DOT;
+    constexpr bool DOT = MODE == GS_DOT || MODE == GS_SUB_DOT;
+    constexpr bool NORM = MODE == GS_SUB_NORM;
     __shared__ double sc[kThreads / 64][2 * kMdMax];
-    double ar[kMdMax], ai[kMdMax];
+    double ar[DOT ? M : 1], ai[DOT ? M : 1];
 #pragma unroll
-    for (int i = 0; i < kMdMax; ++i) ar[i] = ai[i] = 0.0;
+    for (int i = 0; i < (DOT ? M : 1); ++i) ar[i] = ai[i] = 0.0;
+    double nr = 0.0;
+    const double2* V2 = reinterpret_cast<const double2*>(V);
+    double2* w2 = reinterpret_cast<double2*>(w);
     const int64_t beg = blockIdx.x * chunk;
     const int64_t end = min(len, beg + chunk);
     for (int64_t e = beg + threadIdx.x; e < end; e += kThreads) {
-        const double2 wv = reinterpret_cast<const double2*>(w)[e];
+        double2 wv = w2[e];
+        double2 v[M];
 #pragma unroll
-        for (int i = 0; i < kMdMax; ++i) {
-            if (i < m) {
-                const double2 v = reinterpret_cast<const double2*>(V + 2 * len * i)[e];
-                ar[i] = fma(v.x, wv.x, fma(v.y, wv.y, ar[i]));
-                ai[i] = fma(v.x, wv.y, fma(-v.y, wv.x, ai[i]));
+        for (int i = 0; i < M; ++i) v[i] = V2[len * i + e];
+        if constexpr (SUB) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                const double cr = coef[2 * i], ci = coef[2 * i + 1];
+                wv.x = fma(-cr, v[i].x, fma(ci, v[i].y, wv.x));
+                wv.y = fma(-cr, v[i].y, fma(-ci, v[i].x, wv.y));
+            }
+            if constexpr (NORM) {
+                reinterpret_cast<double2*>(out)[e] = wv;
+                nr = fma(wv.x, wv.x, fma(wv.y, wv.y, nr));
+            } else {
+                w2[e] = wv;
+            }
+        }
+        if constexpr (DOT) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                ar[i] = fma(v[i].x, wv.x, fma(v[i].y, wv.y, ar[i]));
+                ai[i] = fma(v[i].x, wv.y, fma(-v[i].y, wv.x, ai[i]));
             }
         }
     }
-    const int wave = threadIdx.x >> 6;
+    if constexpr (DOT || NORM) {
+        constexpr int NS = DOT ? 2 * M : 2;  // slots this mode writes
+        const int wave = threadIdx.x >> 6;
+        const bool lead = (threadIdx.x & 63) == 0;
+        if constexpr (DOT) {
 #pragma unroll
-    for (int i = 0; i < kMdMax; ++i) {
-        if (i < m) {
-            const double r = wsum(ar[i]), q = wsum(ai[i]);
-            if ((threadIdx.x & 63) == 0) {
-                sc[wave][2 * i] = r;
-                sc[wave][2 * i + 1] = q;
+            for (int i = 0; i < M; ++i) {
+                const double r = wsum(ar[i]), q = wsum(ai[i]);
+                if (lead) {
+                    sc[wave][2 * i] = r;
+                    sc[wave][2 * i + 1] = q;
+                }
+            }
+        } else {
+            const double r = wsum(nr);
+            if (lead) {
+                sc[wave][0] = r;
+                sc[wave][1] = 0.0;
             }
         }
+        __syncthreads();
+        if (threadIdx.x < NS) {
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < kThreads / 64; ++k) a += sc[k][threadIdx.x];
+            partials[static_cast<int64_t>(blockIdx.x) * 2 * kMdMax + threadIdx.x] = a;
+        }
     }
+}
+
+// one workgroup per slot: thread t sums workgroups t, t + kThreads, ...; the
+// waves then reduce in a fixed tree, so the result does not depend on timing
+__global__ __launch_bounds__(kThreads) void k_gs_finish(const double* __restrict__ partials, int blocks,
+                                                       double* __restrict__ coef)
+{
+    __shared__ double sw[kThreads / 64];
+    const int slot = blockIdx.x;
+    double a = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) a += partials[static_cast<int64_t>(b) * 2 * kMdMax + slot];
+    a = wsum(a);
+    if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = a;
     __syncthreads();
-    if (threadIdx.x < 2 * m) {
-        double a = 0.0;
-#pragma unroll
-        for (int k = 0; k < kThreads / 64; ++k) a += sc[k][threadIdx.x];
-        partials[static_cast<int64_t>(blockIdx.x) * 2 * kMdMax + threadIdx.x] = a;
+    if (threadIdx.x == 0) coef[slot] = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+}
+
+template <int MODE, int M = 1>
+hipError_t launch_gs_m(int m, int grid, int64_t len, const double* V, double* w, double* out, const double* coef,
+                       int64_t chunk, double* partials, hipStream_t st)
+{
+    if constexpr (M > kMdMax) {
+        return hipErrorInvalidValue;
+    } else {
+        if (m != M) return launch_gs_m<MODE, M + 1>(m, grid, len, V, w, out, coef, chunk, partials, st);
+        hipLaunchKernelGGL((k_gs<M, MODE>), dim3(grid), dim3(kThreads), 0, st, len, V, w, out, coef, chunk, partials);
+        return hipGetLastError();
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_multidot_finish(const double* __restrict__ partials, int blocks, int m,
-                                                             double* __restrict__ coef)
+// chunk of elements per workgroup (at least 4096: 64 KB of each vector) and
+// the grid that covers len with it; grid-strided blocks and larger chunks
+// measured no faster at N=26
+static void gs_grid(int64_t len, int64_t* chunk, int* grid)
 {
-    if (threadIdx.x < 2 * m) {
-        double a = 0.0;
-        for (int b = 0; b < blocks; ++b) a += partials[static_cast<int64_t>(b) * 2 * kMdMax + threadIdx.x];
-        coef[threadIdx.x] = a;
-    }
-}
-
-__global__ void k_gs_sub(int64_t len, int m, const double* __restrict__ V, const double* __restrict__ coef,
-                         double* __restrict__ w)
-{
-    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < len;
-         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        double2 acc = reinterpret_cast<const double2*>(w)[e];
-        for (int i = 0; i < m; ++i) {
-            const double cr = coef[2 * i], ci = coef[2 * i + 1];
-            const double2 v = reinterpret_cast<const double2*>(V + 2 * len * i)[e];
-            acc.x = fma(-cr, v.x, fma(ci, v.y, acc.x));
-            acc.y = fma(-cr, v.y, fma(-ci, v.x, acc.y));
-        }
-        reinterpret_cast<double2*>(w)[e] = acc;
-    }
+    int64_t c = 4096;
+    while ((len + c - 1) / c > kMdGrid) c <<= 1;
+    *chunk = c;
+    *grid = static_cast<int>(std::max<int64_t>(1, (len + c - 1) / c));
 }
 
 size_t multidot_partials_doubles() { return static_cast<size_t>(kMdGrid) * 2 * kMdMax; }
@@ -729,16 +782,35 @@ hipError_t launch_gs_block(int64_t len, int m, const double* V, double* w, doubl
                            hipStream_t st)
 {
     if (m < 1 || m > kMdMax) return hipErrorInvalidValue;
-    int64_t chunk = 4096;
-    while ((len + chunk - 1) / chunk > kMdGrid) chunk <<= 1;
-    const int grid = static_cast<int>(std::max<int64_t>(1, (len + chunk - 1) / chunk));
-    hipLaunchKernelGGL(k_multidot, dim3(grid), dim3(kThreads), 0, st, len, m, V, w, chunk, partials);
-    hipError_t e = hipGetLastError();
+    int64_t chunk;
+    int grid;
+    gs_grid(len, &chunk, &grid);
+    hipError_t e = launch_gs_m<GS_DOT>(m, grid, len, V, w, nullptr, nullptr, chunk, partials, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_multidot_finish, dim3(1), dim3(kThreads), 0, st, partials, grid, m, coef);
+    hipLaunchKernelGGL(k_gs_finish, dim3(2 * m), dim3(kThreads), 0, st, partials, grid, coef);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int sgrid = static_cast<int>(std::min<int64_t>((len + 255) / 256, 8192));
-    hipLaunchKernelGGL(k_gs_sub, dim3(std::max(sgrid, 1)), dim3(256), 0, st, len, m, V, coef, w);
+    return launch_gs_m<GS_SUB>(m, grid, len, V, w, nullptr, coef, chunk, partials, st);
+}
+
+hipError_t launch_cgs2(int64_t len, int m, const double* V, double* w, double* next, double* coef, double* partials,
+                       hipStream_t st)
+{
+    if (m < 1 || m > kMdMax) return hipErrorInvalidValue;
+    int64_t chunk;
+    int grid;
+    gs_grid(len, &chunk, &grid);
+    double* c1 = coef;
+    double* c2 = coef + 2 * kMdMax;
+    hipError_t e = launch_gs_m<GS_DOT>(m, grid, len, V, w, nullptr, nullptr, chunk, partials, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gs_finish, dim3(2 * m), dim3(kThreads), 0, st, partials, grid, c1);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_gs_m<GS_SUB_DOT>(m, grid, len, V, w, nullptr, c1, chunk, partials, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gs_finish, dim3(2 * m), dim3(kThreads), 0, st, partials, grid, c2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (!next) return hipSuccess;
+    if ((e = launch_gs_m<GS_SUB_NORM>(m, grid, len, V, w, next, c2, chunk, partials, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gs_finish, dim3(2), dim3(kThreads), 0, st, partials, grid, coef + 4 * kMdMax);
     return hipGetLastError();
 }
 
@@ -845,6 +917,45 @@ hipError_t launch_copy_probe(const void* src, void* dst, uint64_t bytes, hipStre
     const uint64_t n = bytes / sizeof(float4);
     hipLaunchKernelGGL(k_copy_probe, dim3(16384), dim3(256), 0, st, static_cast<const float4*>(src),
                        static_cast<float4*>(dst), n);
+    return hipGetLastError();
+}
+
+// One-shot copies, one chunk of BS x U float4 per workgroup (no grid-stride
+// loop).  BS = 256, U = 1 with nontemporal accesses: 4 KB workgroups, the
+// fastest read+write stream of the part; consecutive 4 KB chunks go to
+// consecutive workgroups and so round-robin over the XCDs.  BS = 512, U = 16:
+// a workgroup holds 128 KB, the footprint of a 14-qubit complex64 sweep tile.
+typedef float cp4 __attribute__((ext_vector_type(4)));
+template <int BS, int U, bool NT>
+__global__ __launch_bounds__(BS) void k_copy_chunk(const cp4* __restrict__ a, cp4* __restrict__ b)
+{
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * (BS * U) + threadIdx.x;
+    cp4 r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) r[u] = NT ? __builtin_nontemporal_load(a + base + u * BS) : a[base + u * BS];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if constexpr (NT) __builtin_nontemporal_store(r[u], b + base + u * BS);
+        else b[base + u * BS] = r[u];
+    }
+}
+
+uint64_t copy_pattern_chunk(int pattern)
+{
+    return pattern == 1 ? 256ull * 16 : pattern == 2 ? 512ull * 16 * 16 : 16;
+}
+
+hipError_t launch_copy_pattern(int pattern, const void* src, void* dst, uint64_t bytes, hipStream_t st)
+{
+    if (pattern == 0) return launch_copy_probe(src, dst, bytes, st);
+    const uint64_t chunk = copy_pattern_chunk(pattern);
+    if ((pattern != 1 && pattern != 2) || bytes == 0 || bytes % chunk != 0 || bytes / chunk > 0x7fffffffull)
+        return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(bytes / chunk));
+    const cp4* a = static_cast<const cp4*>(src);
+    cp4* b = static_cast<cp4*>(dst);
+    if (pattern == 1) hipLaunchKernelGGL((k_copy_chunk<256, 1, true>), grid, dim3(256), 0, st, a, b);
+    else hipLaunchKernelGGL((k_copy_chunk<512, 16, false>), grid, dim3(512), 0, st, a, b);
     return hipGetLastError();
 }
 

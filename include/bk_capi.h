@@ -385,6 +385,14 @@ int bk_timing_enable(bk_ctx* ctx, int enable);
  * read+write traffic.  The practical HBM ceiling for the sweeps' access mix,
  * and the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE. */
 int bk_copy_probe(bk_ctx* ctx, uint64_t bytes, int reps, double* gbs);
+/* The same measurement in another access pattern (no counterpart in the
+ * reference; a measurement helper).  pattern 0: the grid-stride copy of
+ * bk_copy_probe.  1: one 4 KB chunk per 256-thread workgroup, a grid of
+ * bytes/4096 workgroups, nontemporal loads and stores: the fastest
+ * read+write stream of the part.  2: one 128 KB chunk per 512-thread
+ * workgroup, the footprint of a complex64 14-qubit sweep tile.  bytes must
+ * be a positive multiple of 16, 4096 or 131072 respectively. */
+int bk_copy_probe_pattern(bk_ctx* ctx, uint64_t bytes, int reps, int pattern, double* gbs);
 int bk_timing_read(bk_ctx* ctx, bk_kernel_timing* out, int cap, int* count);
 
 /* ---- the benchmark connector's output files (connector.cuh:839-1155) ----

@@ -95,16 +95,20 @@ def test_lanczos_matches_oracle(engine, orc):
                    np.linalg.norm(vec2[0])) < 1e-6
 
 
-@pytest.mark.parametrize("n", [8, 14])
-def test_krylov_propagator(engine, orc, n):
-    rng = np.random.default_rng(n)
+@pytest.mark.parametrize("n,m", [(8, 30), (14, 30), (14, 32), (14, 33), (12, 45), (12, 2), (4, 30)])
+def test_krylov_propagator(engine, orc, n, m):
+    """Against the oracle's Krylov propagator (same recurrence, full
+    reorthogonalisation).  m <= 32: the one-block fused Gram-Schmidt passes;
+    m = 33, 45: the later steps orthogonalise in blocks of 32; n = 4 (16
+    states) with m = 30: the recurrence breaks down early."""
+    rng = np.random.default_rng(n + 100 * m)
     psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
     psi /= np.linalg.norm(psi)
     engine.upload(psi, C128)
-    engine.krylov_propagate(0.35, 1.0, tau_restart=0.5, m=30)
+    engine.krylov_propagate(0.35, 1.0, tau_restart=0.5, m=m)
     got = engine.download()
-    ref = orc.krylov_propagate(n, 0.35, psi, 0.5, m=30)
-    ref = orc.krylov_propagate(n, 0.35, ref, 0.5, m=30)
+    ref = orc.krylov_propagate(n, 0.35, psi, 0.5, m=m)
+    ref = orc.krylov_propagate(n, 0.35, ref, 0.5, m=m)
     assert np.max(np.abs(got - ref)) < 1e-10
 
 
@@ -120,3 +124,18 @@ def test_errors_are_reported(engine, bk):
         engine.evolve(10, 0, float("nan"), 0.05, 1)
     with pytest.raises(bk.BKError):
         engine.evolve(10, 0, 0.3, 0.05, 1, order=3)
+
+
+def test_copy_probe_patterns(engine, bk):
+    """The HBM copy probes: every pattern copies the same bytes and reports a
+    finite positive rate; sizes off the pattern's chunk are rejected."""
+    nbytes = 1 << 27  # a multiple of every pattern's chunk (16 B, 4 KB, 128 KB)
+    for pattern in (0, 1, 2):
+        gbs = engine.copy_probe(nbytes, 2, pattern=pattern)
+        assert np.isfinite(gbs) and gbs > 0.0, pattern
+    with pytest.raises(bk.BKError):
+        engine.copy_probe(4096 + 16, 1, pattern=1)
+    with pytest.raises(bk.BKError):
+        engine.copy_probe((1 << 20) + 4096, 1, pattern=2)
+    with pytest.raises(bk.BKError):
+        engine.copy_probe(1 << 20, 1, pattern=3)
