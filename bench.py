@@ -134,8 +134,9 @@ def _adam_bench(bk, eng, seed: int, iters: int = 50, lr: float = 0.005) -> dict:
 def _linalg_bench(bk, eng, seed: int) -> dict:
     """The subspace side and sampling: the matrix-free H.v (real FP64 at N=30,
     complex128 at N=28; device tensors, five timed calls after a warm one), one
-    Krylov step of the propagator at N=26 (m=30, complex128), and 100 projective
-    samples of the evolved N=30 complex64 state.  H.v rates count the bytes its
+    Krylov step of the propagator at N=26 (m=30, complex128), 100 projective
+    samples of the evolved N=30 complex64 state, and subspace Lanczos / dense
+    ED on the unique states of 6000 samples of it.  H.v rates count the bytes its
     passes move: each high-qubit group reads x and writes its sum (2 x 8 B per
     real element), the LOW pass reads x, E(s) (2 B) and the groups' sums and
     writes y."""
@@ -180,6 +181,16 @@ def _linalg_bench(bk, eng, seed: int) -> dict:
     u = np.random.default_rng(seed).random(100)
     sec = timed(lambda: eng.sample(u))
     out["sample_n30_c64_100_shots"] = {"ms": sec * 1e3, "gbs_one_read": 8.0 * (1 << n) / sec / 1e9}
+    # the subspace path on a sampled basis: the unique states of 6000 shots (the
+    # reference's default 20 points x 100 shots x 3 seeds) of the state evolved
+    # on to t = 5.2, where it has spread over many basis states
+    eng.evolve(n, s0, 0.35, 0.02, 250, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0, measure_every=0,
+               init=False)
+    basis = np.unique(eng.sample(np.random.default_rng(seed + 1).random(6000))).astype(np.uint64)
+    sl = timed(lambda: eng.subspace_lanczos(n, 0.35, basis, max_krylov_dim=100, k_pairs=2, vectors=False), reps=3)
+    sd = timed(lambda: eng.subspace_dense(n, 0.35, basis, k_pairs=2, vectors=False), reps=1)
+    out["subspace_n30_sampled_basis"] = {"t": 5.2, "shots": 6000, "basis_states": int(len(basis)),
+                                         "lanczos_m100_ms": sl * 1e3, "dense_ms": sd * 1e3}
     return out
 
 
