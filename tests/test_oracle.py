@@ -236,8 +236,8 @@ def test_lie_equals_strang_diagonal_observables(orc):
 
 
 def test_trotter_vs_exact_evolution(orc):
-    """Config 0 shape at N=10: Lie dt=0.05 vs exact exp(-iHt) (numpy eigh);
-    error ~1e-4 and shrinking ~dt for Lie."""
+    """The N=12 dense-ED workload's shape at N=10: Lie dt=0.05 vs exact
+    exp(-iHt) (numpy eigh); the Trotter error is ~1e-4 and shrinks ~dt for Lie."""
     n, g, dt, steps = 10, 0.35, 0.05, 200
     _, s0, e0 = orc.make_trap(n, 4)
     H = orc.dense_h(n, g)
