@@ -225,8 +225,15 @@ template <>
 struct HvCfg<double2> {
     static constexpr int HB = 13, EPU = 1;
 };
+// high passes: 2^9 threads x 16 units; the LOW pass: 2^10 threads x 8 units
+// (its accumulators and E(s) words take 16 units per thread to 256 VGPRs; at
+// 8 units, 4 waves per SIMD hide its two waits on memory per tile: N=30 real
+// FP64 H.v 14.65 -> 13.7 ms, complex128 N=28 7.15 -> 6.56 ms).  1024-thread
+// high passes measured no faster.
 constexpr int kHvUnitBits = 13, kHvThreadBits = 9, kHvRowUnitBits = 4;
 constexpr int kHvThreads = 1 << kHvThreadBits, kHvUPT = 1 << (kHvUnitBits - kHvThreadBits);
+constexpr int kHvLowThreadBits = 10, kHvLowThreads = 1 << kHvLowThreadBits,
+              kHvLowUPT = 1 << (kHvUnitBits - kHvLowThreadBits);
 
 // registers and LDS use a native two-double vector: arrays of it are promoted
 // to registers, where arrays of the HIP vector struct stayed in scratch memory
@@ -246,7 +253,7 @@ __device__ __forceinline__ dv2 d2_axpy(double a, const dv2& x, const dv2& y)
 // LOW pass: y = d(E) x - g sum_{j<HB} x[s ^ 2^j] + (the high groups' sums in
 // acc0, acc1) on contiguous tiles, E(s) from the state-order uint16 cache
 template <typename T, int NACC>
-__global__ __launch_bounds__(kHvThreads) void k_hv_low(const T* __restrict__ x, T* y, uint64_t ntiles, double g,
+__global__ __launch_bounds__(kHvLowThreads) void k_hv_low(const T* __restrict__ x, T* y, uint64_t ntiles, double g,
                                                      double dscale, const uint16_t* __restrict__ ecache,
                                                      const T* acc0, const T* acc1)
 {
@@ -256,33 +263,33 @@ __global__ __launch_bounds__(kHvThreads) void k_hv_low(const T* __restrict__ x, 
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t ubase = tile << kHvUnitBits;  // first unit of the tile
         const dv2* xu = reinterpret_cast<const dv2*>(x) + ubase;
-        dv2 v[kHvUPT];
+        dv2 v[kHvLowUPT];
 #pragma unroll
-        for (int k = 0; k < kHvUPT; ++k) v[k] = xu[(k << kHvThreadBits) + t];
+        for (int k = 0; k < kHvLowUPT; ++k) v[k] = xu[(k << kHvLowThreadBits) + t];
         __syncthreads();  // the previous tile's readers are done
 #pragma unroll
-        for (int k = 0; k < kHvUPT; ++k) lds[(k << kHvThreadBits) + t] = v[k];
+        for (int k = 0; k < kHvLowUPT; ++k) lds[(k << kHvLowThreadBits) + t] = v[k];
         __syncthreads();
         // E(s) of the units' elements (two uint16 for real FP64) and the high
         // groups' sums (written by the high passes), which start the
         // accumulators: read once x's registers are free
-        uint32_t ew[kHvUPT];
+        uint32_t ew[kHvLowUPT];
 #pragma unroll
-        for (int k = 0; k < kHvUPT; ++k) {
-            const uint64_t gu = ubase + (k << kHvThreadBits) + t;
+        for (int k = 0; k < kHvLowUPT; ++k) {
+            const uint64_t gu = ubase + (k << kHvLowThreadBits) + t;
             ew[k] = EPU == 2 ? reinterpret_cast<const uint32_t*>(ecache)[gu] : static_cast<uint32_t>(ecache[gu]);
         }
-        dv2 ad[kHvUPT];
+        dv2 ad[kHvLowUPT];
 #pragma unroll
-        for (int k = 0; k < kHvUPT; ++k) {
+        for (int k = 0; k < kHvLowUPT; ++k) {
             ad[k] = mkv(0.0, 0.0);
             if constexpr (NACC >= 1) {
-                const dv2 w = reinterpret_cast<const dv2*>(acc0)[ubase + (k << kHvThreadBits) + t];
+                const dv2 w = reinterpret_cast<const dv2*>(acc0)[ubase + (k << kHvLowThreadBits) + t];
                 ad[k].x += w.x;
                 ad[k].y += w.y;
             }
             if constexpr (NACC >= 2) {
-                const dv2 w = reinterpret_cast<const dv2*>(acc1)[ubase + (k << kHvThreadBits) + t];
+                const dv2 w = reinterpret_cast<const dv2*>(acc1)[ubase + (k << kHvLowThreadBits) + t];
                 ad[k].x += w.x;
                 ad[k].y += w.y;
             }
@@ -295,9 +302,9 @@ __global__ __launch_bounds__(kHvThreads) void k_hv_low(const T* __restrict__ x, 
         // keeps each unit's LDS reads after the previous unit's, so they are
         // not all hoisted into registers at once.
 #pragma unroll
-        for (int k = 0; k < kHvUPT; ++k) {
+        for (int k = 0; k < kHvLowUPT; ++k) {
             __builtin_amdgcn_sched_barrier(0);
-            const uint32_t u = (k << kHvThreadBits) + t;
+            const uint32_t u = (k << kHvLowThreadBits) + t;
             const dv2 self = lds[u];
 #pragma unroll
             for (int j = 0; j < kHvUnitBits; ++j) ad[k] = d2_axpy(-g, lds[u ^ (1u << j)], ad[k]);
@@ -412,15 +419,15 @@ static hipError_t launch_hv_low(int nacc, const T* x, T* y, uint64_t ntiles, dou
 {
     switch (nacc) {
     case 0:
-        hipLaunchKernelGGL((k_hv_low<T, 0>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, g, dscale, ecache,
+        hipLaunchKernelGGL((k_hv_low<T, 0>), dim3(grid), dim3(kHvLowThreads), 0, st, x, y, ntiles, g, dscale, ecache,
                            acc0, acc1);
         break;
     case 1:
-        hipLaunchKernelGGL((k_hv_low<T, 1>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, g, dscale, ecache,
+        hipLaunchKernelGGL((k_hv_low<T, 1>), dim3(grid), dim3(kHvLowThreads), 0, st, x, y, ntiles, g, dscale, ecache,
                            acc0, acc1);
         break;
     default:
-        hipLaunchKernelGGL((k_hv_low<T, 2>), dim3(grid), dim3(kHvThreads), 0, st, x, y, ntiles, g, dscale, ecache,
+        hipLaunchKernelGGL((k_hv_low<T, 2>), dim3(grid), dim3(kHvLowThreads), 0, st, x, y, ntiles, g, dscale, ecache,
                            acc0, acc1);
         break;
     }
