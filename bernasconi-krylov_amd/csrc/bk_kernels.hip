@@ -1043,6 +1043,10 @@ struct Cfg<double2> {
     static constexpr int HTB = 13, HRB = 4;
 };
 
+static_assert(LowPlan<Cfg<float2>::TB, Cfg<float2>::RB>::plan.TT() == kThreadBits &&
+                  LowPlan<Cfg<double2>::TB, Cfg<double2>::RB>::plan.TT() == kThreadBits,
+              "the default LOW tiles run kThreads threads (the reduction scratch and grids assume it)");
+
 struct MixCoef {
     bool tan_form;
     double bc, bs, scale;
@@ -1084,8 +1088,9 @@ template <typename V, int FLAGS>
 hipError_t launch_low(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FLAGS, C::SW>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, psi, a);
+    using PL = LowPlan<C::TB, C::RB>;
+    auto k = k_sweep<V, PL, FLAGS, C::SW>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PL::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
 }
 
@@ -1280,8 +1285,9 @@ template <typename V, int FL = F_PHASE | F_GRAD | F_ECACHE>
 hipError_t launch_low_rev(V* phi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW, 2>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, phi, a);
+    using PL = LowPlan<C::TB, C::RB>;
+    auto k = k_sweep<V, PL, FL, C::SW, 2>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PL::plan.TT()), 0, st, phi, a);
     return hipGetLastError();
 }
 
@@ -1389,8 +1395,9 @@ template <typename V, int FL>
 hipError_t launch_low_batch(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    auto k = k_sweep<V, LowPlan<C::TB, C::RB>, FL, C::SW>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, st, psi, a);
+    using PL = LowPlan<C::TB, C::RB>;
+    auto k = k_sweep<V, PL, FL, C::SW>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PL::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
 }
 
