@@ -1960,7 +1960,6 @@ hipError_t phase_from_impl(V* dst, const V* src, int n, const void* tab, double*
 
 SweepTuning default_tuning(int n, int precision)
 {
-    (void)n;
     SweepTuning t{};
     t.tile_bits = precision == kC64 ? kLowTB64 : kLowTB128;
     t.row_bits = precision == kC64 ? 5 : 4;
@@ -1969,7 +1968,10 @@ SweepTuning default_tuning(int n, int precision)
     t.wide_low = true;
     t.prefetch_low = false;
     t.prefetch_high = false;
-    t.page_bits = precision == kC64 ? 20 : 19;
+    // page-balancing granule of the high groups' planner: measured best at 2^20
+    // amplitudes through N=30 (7.40 ms against 7.53-7.58 ms per step at 2^22-2^24)
+    // and at 2^24 from N=31 (15.50 -> 15.16 ms; N=32 33.0 -> 31.8 ms)
+    t.page_bits = precision == kC64 ? (n >= 31 ? 24 : 20) : 19;
     return t;
 }
 
