@@ -1968,10 +1968,12 @@ SweepTuning default_tuning(int n, int precision)
     t.wide_low = true;
     t.prefetch_low = false;
     t.prefetch_high = false;
-    // page-balancing granule of the high groups' planner: measured best at 2^20
-    // amplitudes through N=30 (7.40 ms against 7.53-7.58 ms per step at 2^22-2^24)
-    // and at 2^24 from N=31 (15.50 -> 15.16 ms; N=32 33.0 -> 31.8 ms)
-    t.page_bits = precision == kC64 ? (n >= 31 ? 24 : 20) : 19;
+    // page-balancing granule of the high groups' planner, in amplitudes: 2^20
+    // through N=30, 2^24 from N=31.  Complex64: N=30 7.40 ms per step against
+    // 7.53-7.58 ms at 2^22-2^24; N=31 15.50 -> 15.16 ms, N=32 33.0 -> 31.8 ms.
+    // Complex128: 2^19 and 2^20 equal at N=26-28; N=30 15.24 -> 14.84 ms,
+    // N=31 35.2 -> 31.6 ms, N=32 71.7 -> 67.7 ms.
+    t.page_bits = n >= 31 ? 24 : 20;
     return t;
 }
 
