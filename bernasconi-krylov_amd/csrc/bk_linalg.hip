@@ -802,7 +802,12 @@ hipError_t launch_gs_block(int64_t len, int m, const double* V, double* w, doubl
 hipError_t launch_cgs2(int64_t len, int m, const double* V, double* w, double* next, double* coef, double* partials,
                        hipStream_t st)
 {
-    if (m < 1 || m > kMdMax) return hipErrorInvalidValue;
+    if (m < 1 || m > kMdMax || len < 1) return hipErrorInvalidValue;
+    if (next) {
+        // the last pass reads w and the basis while it writes next
+        const double* vend = V + 2 * len * static_cast<int64_t>(m);
+        if (next == w || (next + 2 * len > V && next < vend)) return hipErrorInvalidValue;
+    }
     int64_t chunk;
     int grid;
     gs_grid(len, &chunk, &grid);
