@@ -37,8 +37,11 @@ def test_sampling_inverse_cdf(engine, orc, n, prec):
     assert len(bad) <= 2
 
 
-@pytest.mark.parametrize("n", [4, 9, 12, 13, 17, 22])
+@pytest.mark.parametrize("n", [4, 9, 12, 13, 14, 17, 22, 24])
 def test_hv_parity(engine, orc, n):
+    """Against the oracle's H.v.  The tiled LOW pass adds 0, 1 or 2 high-group
+    sums: n = 14 (real FP64: none; complex128: one), 17 and 22 (one), 24
+    (two in both precisions)."""
     import torch
     print(f"hv n={n}", flush=True)
     rng = np.random.default_rng(n)
