@@ -166,6 +166,8 @@ _SIGS = {
     "bk_count_below": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_uint64)]),
     "bk_sample": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
                             C.POINTER(C.c_int64)]),
+    "bk_sample_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_void_p]),
     "bk_hv": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_void_p,
                         C.c_void_p, C.c_void_p]),
     "bk_lanczos": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,

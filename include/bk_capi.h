@@ -224,6 +224,19 @@ int bk_count_below(bk_ctx* ctx, int n, int64_t e0, uint64_t* count);
  * out_idx[i] = min{ s : cdf(s) > randnums[i] * cdf(2^n - 1) } on the held state. */
 int bk_sample(bk_ctx* ctx, const double* randnums, int64_t shots, int64_t* out_idx);
 
+/* No counterpart in the reference: bk_sample for callers that cannot allocate
+ * device memory (work under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and results as bk_sample, on a caller-held
+ * device state d_amps (2^n amplitudes in 'precision', interleaved re,im), with
+ * device random numbers d_randnums[shots] and device output d_out_idx[shots];
+ * launched on hip_stream (NULL: the null stream), which must belong to the
+ * current device, and not synchronised.  No device memory is allocated: the
+ * block sums live in a library-held static array of 16 MiB per device (n <= 32;
+ * larger n, shots > 2^30 or null pointers return BK_E_INVALID).  Every call on
+ * one device shares that array, so calls that overlap on one device race. */
+int bk_sample_noalloc(int n, int precision, const void* d_amps, const double* d_randnums, int64_t shots,
+                      int64_t* d_out_idx, void* hip_stream);
+
 /* ---- matrix-free H.v (diagonalizer.cuh:60-77; the MatVecFunc of :400-406) ----
  * y = (E/N or E) * x - g * sum_j X_j x on DEVICE pointers; is_complex selects
  * complex128 (interleaved) over FP64 real.  stream may be NULL (context stream). */
