@@ -124,6 +124,21 @@ class _MinimumFamilyC(C.Structure):
                 ("energy", C.c_int64), ("times_visited", C.c_uint64), ("evals_to_first_find", C.c_uint64)]
 
 
+BK_MINIMA_MAX_LEVELS = 8
+
+
+class _MinimaOrbitC(C.Structure):
+    _fields_ = [("canonical_state", C.c_uint64), ("energy", C.c_int64), ("level", C.c_int), ("d_min", C.c_int),
+                ("sd_hit_fraction", C.c_double)]
+
+
+class _ExhaustiveResultC(C.Structure):
+    _fields_ = [("n", C.c_int), ("e_star", C.c_int64), ("total_minima_found", C.c_int64),
+                ("total_orbits_found", C.c_int64), ("num_energies", C.c_int64), ("num_levels", C.c_int),
+                ("level_energy", C.c_int64 * BK_MINIMA_MAX_LEVELS), ("level_orbits", C.c_int64 * BK_MINIMA_MAX_LEVELS),
+                ("orbits_written", C.c_int64), ("search_time_sec", C.c_double)]
+
+
 class _TrapRecordC(C.Structure):
     _fields_ = [("n", C.c_int), ("g", C.c_double), ("trap_rank", C.c_int64), ("trap_bits", C.c_uint64),
                 ("canonical_bits", C.c_uint64), ("trap_energy", C.c_int64), ("d_min", C.c_int),
@@ -196,6 +211,12 @@ _SIGS = {
     "bk_collect_minima": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                                     C.POINTER(_MinimumFamilyC), C.c_int64, C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "bk_d_min_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.c_int64,
+                                 C.c_int, C.POINTER(C.c_int)]),
+    "bk_exhaustive_minima": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+                                       C.POINTER(_ExhaustiveResultC), C.POINTER(_MinimaOrbitC), C.c_int64]),
+    "bk_export_exhaustive_json": (C.c_int, [C.c_char_p, C.POINTER(_ExhaustiveResultC), C.POINTER(_MinimaOrbitC),
+                                            C.c_int64]),
     "bk_export_results_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
     "bk_export_time_series_csv": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int]),
     "bk_export_structured_json": (C.c_int, [C.c_char_p, C.POINTER(_TrapRecordC), C.c_int, C.c_int]),
@@ -573,6 +594,37 @@ class Engine:
                                        target_families, C.byref(cnt), C.byref(starts), C.byref(evals)))
         fams = [{f: getattr(out[i], f) for f, _ in _MinimumFamilyC._fields_} for i in range(cnt.value)]
         return {"families": fams, "total_starts": starts.value, "total_evals": evals.value}
+
+    def d_min_batch(self, n: int, states, targets=None, max_d: int = -1) -> np.ndarray:
+        """LabsEvaluator::compute_d_min (labs_energy.hpp:175-207) of many states on the
+        GPU; targets None (or entries < 0): each state's own energy."""
+        st = np.ascontiguousarray(np.asarray(states, dtype=np.uint64))
+        k = st.shape[0]
+        tg = None if targets is None else np.ascontiguousarray(np.asarray(targets, dtype=np.int64))
+        out = np.zeros(k, np.int32)
+        tp = tg.ctypes.data_as(C.POINTER(C.c_int64)) if tg is not None else None
+        _check(lib().bk_d_min_batch(self._h, n, st.ctypes.data_as(C.POINTER(C.c_uint64)), tp, k, max_d,
+                                    out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def exhaustive_minima(self, n: int, levels: int = 2, num_sd_starts: int = 0, sd_seed: int = 42,
+                          cap: int = 1 << 16) -> dict:
+        """run_exploration (exhaustive_minima.cpp:109-216) on the GPU: E*, the minima
+        and orbit totals, and the trap levels with every orbit's canonical state,
+        exact d_min and (with num_sd_starts > 0) sd_hit_fraction."""
+        res = _ExhaustiveResultC()
+        orb = (_MinimaOrbitC * cap)()
+        _check(lib().bk_exhaustive_minima(self._h, n, levels, num_sd_starts, sd_seed, C.byref(res), orb, cap))
+        lv = []
+        for i in range(res.num_levels):
+            lv.append({"E0": res.level_energy[i], "num_orbits": res.level_orbits[i], "orbits": []})
+        for i in range(res.orbits_written):
+            o = orb[i]
+            lv[o.level - 1]["orbits"].append({"canonical": o.canonical_state, "d_min": o.d_min,
+                                              "sd_hit_fraction": o.sd_hit_fraction})
+        return {"n": res.n, "e_star": res.e_star, "total_minima_found": res.total_minima_found,
+                "total_orbits_found": res.total_orbits_found, "num_energies": res.num_energies,
+                "levels": lv, "search_time_sec": res.search_time_sec}
 
     def _basis(self, basis) -> np.ndarray:
         return np.ascontiguousarray(np.asarray(basis, dtype=np.uint64))

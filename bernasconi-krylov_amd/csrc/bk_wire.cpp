@@ -89,4 +89,31 @@ int bk_export_structured_json(const char* path, const bk_trap_record* recs, int 
     return bkc::guard_abi([&] { return bk_export_structured_json_impl(path, recs, count, n_override); });
 }
 
+static int bk_export_exhaustive_json_impl(const char* path, const bk_exhaustive_result* res,
+                                          const bk_minima_orbit* orbits, int64_t count)
+{
+    if (!res || count < 0 || (count > 0 && !orbits) || res->num_levels < 0 ||
+        res->num_levels > BK_MINIMA_MAX_LEVELS || res->n < 1 || res->n > 64)
+        return bkc::fail(BK_E_INVALID, "bk_export_exhaustive_json: bad argument");
+    bk::wire::ExhaustiveMinima r;
+    r.n = res->n;
+    r.e_star = res->e_star;
+    r.levels.resize(static_cast<size_t>(res->num_levels));
+    for (int l = 0; l < res->num_levels; ++l) r.levels[static_cast<size_t>(l)].energy = res->level_energy[l];
+    for (int64_t i = 0; i < count; ++i) {
+        const bk_minima_orbit& o = orbits[i];
+        if (o.level < 1 || o.level > res->num_levels)
+            return bkc::fail(BK_E_INVALID, "bk_export_exhaustive_json: orbit %lld has level %d",
+                             static_cast<long long>(i), o.level);
+        r.levels[static_cast<size_t>(o.level - 1)].orbits.push_back({o.canonical_state, o.d_min, o.sd_hit_fraction});
+    }
+    return guarded(path, [&] { bk::wire::export_exhaustive_json(path, r); });
+}
+
+int bk_export_exhaustive_json(const char* path, const bk_exhaustive_result* res, const bk_minima_orbit* orbits,
+                              int64_t count)
+{
+    return bkc::guard_abi([&] { return bk_export_exhaustive_json_impl(path, res, orbits, count); });
+}
+
 }  // extern "C"

@@ -114,3 +114,25 @@ def export_structured_json(path: str, records: List[TrapRecord], n_override: int
     """export_structured_json (connector.cuh:935-1155)."""
     arr, _keep = _records(records)
     _bk._check(_bk.lib().bk_export_structured_json(str(path).encode(), arr, len(records), int(n_override)))
+
+
+def export_exhaustive_json(path: str, result: dict) -> None:
+    """save_to_json (exhaustive_minima.cpp:328-372) of a result shaped like
+    Engine.exhaustive_minima's: {"n", "e_star", "levels": [{"E0", "orbits":
+    [{"canonical", "d_min", "sd_hit_fraction"}]}]}."""
+    levels = result["levels"]
+    if len(levels) > _bk.BK_MINIMA_MAX_LEVELS:
+        raise ValueError("too many levels")
+    res = _bk._ExhaustiveResultC()
+    res.n = int(result["n"])
+    res.e_star = int(result["e_star"])
+    res.num_levels = len(levels)
+    orbs = []
+    for i, lv in enumerate(levels):
+        res.level_energy[i] = int(lv["E0"])
+        res.level_orbits[i] = len(lv["orbits"])
+        for o in lv["orbits"]:
+            orbs.append(_bk._MinimaOrbitC(int(o["canonical"]), int(lv["E0"]), i + 1, int(o["d_min"]),
+                                          float(o.get("sd_hit_fraction", 0.0))))
+    arr = (_bk._MinimaOrbitC * max(1, len(orbs)))(*orbs)
+    _bk._check(_bk.lib().bk_export_exhaustive_json(path.encode(), C.byref(res), arr, len(orbs)))

@@ -10,6 +10,7 @@
 //   export_results_csv ....... connector.cuh:839-904
 //   export_time_series_csv ... connector.cuh:906-933
 //   export_structured_json ... connector.cuh:935-1155
+//   export_exhaustive_json ... exhaustive_minima.cpp:328-372 (the explorer's file)
 //
 // Byte for byte, stream state included.  The reference writes each file
 // through one std::ofstream and sets a float format only on some columns, so
@@ -423,6 +424,50 @@ inline void export_structured_json(const std::string& path, const std::vector<Tr
             c.txt("          }\n        }").txt(gi + 1 < gs.size() ? "," : "").txt("\n");
         }
         c.txt("      }\n    }").txt(ti + 1 < traps.size() ? "," : "").txt("\n");
+    }
+    c.txt("  ]\n}\n");
+}
+
+// The local-minima explorer's result (ExhaustiveResult / LevelInfo /
+// OrbitInfo, exhaustive_minima.cpp:29-50): the trap levels above E* and their
+// orbits, each by its canonical state.
+struct MinimaOrbit {
+    uint64_t canonical_bits = 0;
+    int d_min = 0;
+    double sd_hit_fraction = 0.0;
+};
+struct MinimaLevel {
+    int64_t energy = 0;
+    std::vector<MinimaOrbit> orbits;
+};
+struct ExhaustiveMinima {
+    int n = 0;
+    int64_t e_star = 0;
+    std::vector<MinimaLevel> levels;
+};
+
+// save_to_json (exhaustive_minima.cpp:328-372): canonical_id is "0x" and
+// upper-case hex, the bit string is n characters with the most significant
+// spin first, sd_hit_fraction is fixed with 6 decimals.
+inline void export_exhaustive_json(const std::string& path, const ExhaustiveMinima& r)
+{
+    std::ofstream f = detail::open_out(path, "cannot open exhaustive-minima JSON: ");
+    Col c(f);
+    c.txt("{\n  \"N\": ").keep(r.n).txt(",\n  \"E_star\": ").keep(r.e_star).txt(",\n  \"levels\": [\n");
+    for (size_t i = 0; i < r.levels.size(); ++i) {
+        const MinimaLevel& lv = r.levels[i];
+        c.txt("    {\n      \"E0\": ").keep(lv.energy).txt(",\n      \"orbits\": [\n");
+        for (size_t j = 0; j < lv.orbits.size(); ++j) {
+            const MinimaOrbit& o = lv.orbits[j];
+            TrapBenchmark ids;
+            set_canonical(ids, o.canonical_bits, r.n);
+            c.txt("        {\n          \"canonical_id\": \"").txt(ids.canonical_id);
+            c.txt("\",\n          \"bitstring\": \"").txt(ids.canonical_bitstring);
+            c.txt("\",\n          \"d_min\": ").keep(o.d_min);
+            c.txt(",\n          \"sd_hit_fraction\": ").fix(o.sd_hit_fraction, 6);
+            c.txt("\n        }").txt(j + 1 < lv.orbits.size() ? "," : "").txt("\n");
+        }
+        c.txt("      ]\n    }").txt(i + 1 < r.levels.size() ? "," : "").txt("\n");
     }
     c.txt("  ]\n}\n");
 }

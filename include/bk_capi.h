@@ -99,6 +99,57 @@ int bk_collect_minima(bk_ctx* ctx, int n, uint64_t seed, int64_t target_families
                       int64_t max_starts, bk_minimum_family* out, int64_t cap, int64_t* count, int64_t* total_starts,
                       int64_t* total_evals);
 
+/* compute_d_min (labs_energy.hpp:175-207) for count states on the device:
+ * out[i] = the smallest Hamming radius d <= max_d (all radii when max_d <= 0)
+ * at which some state has E < targets[i] (targets NULL or targets[i] < 0:
+ * E(states[i])); max_d + 1 (n when max_d <= 0) when there is none or the
+ * target is at most E*(n).  The value of bk_d_min for every state.
+ * 2 <= n <= 32, count <= 2^24. */
+int bk_d_min_batch(bk_ctx* ctx, int n, const uint64_t* states, const int64_t* targets, int64_t count, int max_d,
+                   int* out);
+
+/* ---- exhaustive local-minima explorer (src/tools/exhaustive_minima.cpp) ----
+ * run_exploration (:109-216): scans all 2^(n-1) states with the top spin up
+ * (global negation gives the rest) for one-flip minima (no single flip lowers
+ * E, :74-107), finds E* (the lowest energy of a minimum) and the next
+ * 'levels' energies that hold minima (the trap levels; the reference keeps
+ * 2), and lists every symmetry orbit of minima at those levels by its
+ * canonical state with its exact d_min at the level energy (:200-203).  With
+ * num_sd_starts > 0, steepest descents (bk_descent_batch) from the first
+ * num_sd_starts draws of mt19937_64(sd_seed) & (2^n - 1) give each listed
+ * orbit's sd_hit_fraction, the share of descents ending in it
+ * (run_sd_benchmarks, :218-271; the reference's CLI uses seed 42 and 100000
+ * starts).  3 <= n <= 32. */
+#define BK_MINIMA_MAX_LEVELS 8
+typedef struct {
+    uint64_t canonical_state;  /* SpinState::canonical of the orbit */
+    int64_t energy;            /* the level energy E0 */
+    int level;                 /* 1-based trap level */
+    int d_min;                 /* compute_d_min(canonical_state, E0) */
+    double sd_hit_fraction;    /* hits / num_sd_starts (0 without descents) */
+} bk_minima_orbit;
+
+typedef struct {               /* ExhaustiveResult (:43-50) */
+    int n;
+    int64_t e_star;              /* -1 when no minimum exists */
+    int64_t total_minima_found;  /* minima over all 2^n states: twice the half-space count (:151) */
+    int64_t total_orbits_found;  /* distinct canonical orbits over all energies */
+    int64_t num_energies;        /* energies that hold a minimum */
+    int num_levels;              /* trap levels found (<= levels) */
+    int64_t level_energy[BK_MINIMA_MAX_LEVELS];
+    int64_t level_orbits[BK_MINIMA_MAX_LEVELS];
+    int64_t orbits_written;      /* entries written to orbits: level by level, ascending canonical state */
+    double search_time_sec;      /* device time of the scan */
+} bk_exhaustive_result;
+
+int bk_exhaustive_minima(bk_ctx* ctx, int n, int levels, uint64_t num_sd_starts, uint64_t sd_seed,
+                         bk_exhaustive_result* res, bk_minima_orbit* orbits, int64_t cap);
+
+/* save_to_json (exhaustive_minima.cpp:328-372): the explorer's JSON file, byte
+ * for byte the reference writer's for the same values.  Host-only. */
+int bk_export_exhaustive_json(const char* path, const bk_exhaustive_result* res, const bk_minima_orbit* orbits,
+                              int64_t count);
+
 /* ---- statevector held by the context ---- */
 /* allocate 2^n amplitudes in 'precision' and set |start_bits> (sampler.cuh:428-438) */
 int bk_state_init(bk_ctx* ctx, int n, int precision, uint64_t start_bits);

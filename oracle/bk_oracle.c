@@ -1057,3 +1057,132 @@ int bko_schedule(double t_max, double dt, int num_sample_points, int shots_per_p
     }
     return total;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Exhaustive one-flip local minima: exhaustive_minima.cpp:74-208.           */
+/* ------------------------------------------------------------------------ */
+/* is_one_flip_local_minimum (exhaustive_minima.cpp:74-107): s is a minimum
+ * when no single flip lowers E.  With C_k the autocorrelations of s, flipping
+ * spin p changes C_k by d_k = -2 s_p (s_{p+k} + s_{p-k}) over the pairs present
+ * and E by sum_k d_k (2 C_k + d_k); the scan stops at the first flip whose
+ * change is negative. */
+static int one_flip_minimum(int n, uint64_t s, const int64_t *C)
+{
+    for (int p = 0; p < n; ++p) {
+        const int sp = ((s >> p) & 1ULL) ? -1 : 1;
+        int64_t de = 0;
+        for (int k = 1; k < n; ++k) {
+            int d = 0;
+            if (p + k < n) d -= 2 * sp * (((s >> (p + k)) & 1ULL) ? -1 : 1);
+            if (p - k >= 0) d -= 2 * (((s >> (p - k)) & 1ULL) ? -1 : 1) * sp;
+            de += (int64_t)d * (2 * C[k] + d);
+        }
+        if (de < 0) return 0;
+    }
+    return 1;
+}
+
+typedef struct {
+    int64_t e;
+    uint64_t c;
+} bko_min_rec;
+
+static int cmp_min_rec(const void *a, const void *b)
+{
+    const bko_min_rec *x = (const bko_min_rec *)a, *y = (const bko_min_rec *)b;
+    if (x->e != y->e) return x->e < y->e ? -1 : 1;
+    if (x->c != y->c) return x->c < y->c ? -1 : 1;
+    return 0;
+}
+
+/* run_exploration (exhaustive_minima.cpp:109-208): every s in [0, 2^(n-1))
+ * (the other half follows by global negation) that is a one-flip minimum adds
+ * its canonical image (SpinState::canonical) to the set of its energy.  The
+ * lowest energy is E*; the next 'levels' energies are the trap levels, whose
+ * orbits are listed in ascending canonical order (std::set order). */
+int bko_exhaustive_minima(int n, int levels, int64_t *e_star, int64_t *total_minima, int64_t *total_orbits,
+                          int64_t *num_energies, int64_t *level_energy, int64_t *level_orbits, uint64_t *orbits,
+                          int64_t cap)
+{
+    if (n < 2 || n > 40 || levels < 0) return -1;
+    const uint64_t half = 1ULL << (n - 1);
+    int64_t count = 0, alloc = 1024;
+    bko_min_rec *recs = (bko_min_rec *)malloc(sizeof(bko_min_rec) * (size_t)alloc);
+    if (!recs) return -1;
+    int failed = 0;
+#pragma omp parallel
+    {
+        int64_t lc = 0, la = 256;
+        bko_min_rec *loc = (bko_min_rec *)malloc(sizeof(bko_min_rec) * (size_t)la);
+        int64_t C[64];
+#pragma omp for schedule(dynamic, 65536)
+        for (uint64_t s = 0; s < half; ++s) {
+            bko_ck(n, s, C);
+            if (!one_flip_minimum(n, s, C)) continue;
+            int64_t e = 0;
+            for (int k = 1; k < n; ++k) e += C[k] * C[k];
+            if (lc == la) {
+                la *= 2;
+                bko_min_rec *t = loc ? (bko_min_rec *)realloc(loc, sizeof(bko_min_rec) * (size_t)la) : NULL;
+                if (!t) {
+#pragma omp atomic write
+                    failed = 1;
+                    continue;
+                }
+                loc = t;
+            }
+            if (loc) {
+                loc[lc].e = e;
+                loc[lc].c = bko_canonical(n, s);
+                ++lc;
+            }
+        }
+#pragma omp critical
+        {
+            if (count + lc > alloc) {
+                while (count + lc > alloc) alloc *= 2;
+                bko_min_rec *t = (bko_min_rec *)realloc(recs, sizeof(bko_min_rec) * (size_t)alloc);
+                if (t) recs = t;
+                else failed = 1;
+            }
+            if (!failed && lc > 0) {
+                memcpy(recs + count, loc, sizeof(bko_min_rec) * (size_t)lc);
+                count += lc;
+            }
+        }
+        free(loc);
+    }
+    if (failed) {
+        free(recs);
+        return -1;
+    }
+    qsort(recs, (size_t)count, sizeof(bko_min_rec), cmp_min_rec);
+    *total_minima = 2 * count;
+    int64_t orb = 0, ne = 0, written = 0;
+    int lvl = -1; /* -1: E*, 0.. : trap levels */
+    for (int64_t i = 0; i < count; ++i) {
+        const int newe = (i == 0 || recs[i].e != recs[i - 1].e);
+        const int newc = newe || recs[i].c != recs[i - 1].c;
+        if (newe) {
+            ++ne;
+            if (ne == 1) *e_star = recs[i].e;
+            lvl = (int)ne - 2;
+            if (lvl >= 0 && lvl < levels) {
+                level_energy[lvl] = recs[i].e;
+                level_orbits[lvl] = 0;
+            }
+        }
+        if (!newc) continue;
+        ++orb;
+        if (lvl >= 0 && lvl < levels) {
+            ++level_orbits[lvl];
+            if (written < cap) orbits[written++] = recs[i].c;
+        }
+    }
+    *total_orbits = orb;
+    *num_energies = ne;
+    free(recs);
+    if (count == 0) *e_star = -1;
+    const int64_t lv = ne - 1;
+    return (int)(lv < levels ? (lv > 0 ? lv : 0) : levels);
+}

@@ -13,6 +13,9 @@
  *   bko_known_optimal_energy   brute-force min over all 2^N (N <= 24 in tests)
  *   bko_d_min                  brute force over all states (tests), oracle/_ref,
  *                              exhaustive_minima_n*.json d_min values
+ *   bko_exhaustive_minima      exhaustive_minima_n{20,22,24}.json (E*, trap energies,
+ *                              canonical orbit ids), brute force over all states and
+ *                              their neighbours at n <= 14 (tests)
  *   bko_random_state /
  *   bko_steepest_descent       oracle/_ref (reference classical.cuh compiled here),
  *                              seed fixtures in tests/golden
@@ -58,6 +61,19 @@ double bko_mt64_uniform(bko_mt64 *g, double a, double b);
 uint64_t bko_random_state(int n, uint64_t seed);
 void bko_steepest_descent(int n, uint64_t start, uint64_t *final_state,
                           int64_t *final_energy, uint64_t *evals, uint64_t *steps);
+
+/* ---- exhaustive one-flip local minima: exhaustive_minima.cpp:74-208 ----
+ * Scans s in [0, 2^(n-1)) for one-flip minima and groups their canonical
+ * images by energy.  Writes E* (lowest energy with a minimum; -1 if none),
+ * total_minima (2 x the half-space count, as the reference reports it),
+ * total_orbits (distinct canonical states over all energies), num_energies,
+ * and for the trap levels l = 0..levels-1 (the energies after E*)
+ * level_energy[l], level_orbits[l] and their canonical states in ascending
+ * order, level by level, into orbits (at most cap).  Returns the number of
+ * trap levels found (<= levels), -1 on bad arguments or allocation failure. */
+int bko_exhaustive_minima(int n, int levels, int64_t *e_star, int64_t *total_minima, int64_t *total_orbits,
+                          int64_t *num_energies, int64_t *level_energy, int64_t *level_orbits, uint64_t *orbits,
+                          int64_t cap);
 
 /* ---- FP64 statevector Trotter loop: sampler.cuh:54-227, 412-619 ----
  * order: 1 = Lie, phase first (U = X P, the reference's loop);

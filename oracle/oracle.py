@@ -96,6 +96,9 @@ def lib():
         L.bko_schedule.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
                                    C.c_double, C.c_int, C.c_int, C.c_double,
                                    C.POINTER(C.c_ubyte), C.c_int]
+        L.bko_exhaustive_minima.restype = C.c_int
+        L.bko_exhaustive_minima.argtypes = [C.c_int, C.c_int, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p,
+                                            C.POINTER(_u64), _i64]
         _lib = L
     return _lib
 
@@ -118,6 +121,9 @@ def ref():
         R.bkref_random_state.argtypes = [C.c_int, _u64]
         R.bkref_steepest_descent.argtypes = [C.c_int, _u64, C.POINTER(_u64), _i64p,
                                              C.POINTER(_u64), C.POINTER(_u64)]
+        R.bkref_exhaustive_minima.restype = C.c_int
+        R.bkref_exhaustive_minima.argtypes = [C.c_int, C.c_int, _u64, _i64p, _i64p, _i64p, _i64p, _i64p,
+                                              C.POINTER(_u64), C.POINTER(C.c_int), _dp, _i64, C.c_char_p]
         _ref = R
     return _ref
 
@@ -352,3 +358,52 @@ def schedule(t_max, dt, num_sample_points=20, shots_per_point=100, t_start=0.0,
                        1 if track_p_less else 0, 1 if record_trajectory else 0, dt_trajectory,
                        flags.ctypes.data_as(C.POINTER(C.c_ubyte)), flags.shape[0])
     return flags
+
+
+# ---------------------------------------------------------------- exhaustive minima
+def exhaustive_minima(n: int, levels: int = 2, cap: int = 1 << 16) -> dict:
+    """run_exploration (exhaustive_minima.cpp:109-208) without d_min: E*, the
+    reported minima and orbit totals, and the trap levels' energies and
+    canonical orbits (ascending)."""
+    es, tm, to, ne = _i64(), _i64(), _i64(), _i64()
+    le = np.zeros(max(levels, 1), dtype=np.int64)
+    lo = np.zeros(max(levels, 1), dtype=np.int64)
+    orb = np.zeros(cap, dtype=np.uint64)
+    k = lib().bko_exhaustive_minima(n, levels, C.byref(es), C.byref(tm), C.byref(to), C.byref(ne),
+                                    le.ctypes.data_as(_i64p), lo.ctypes.data_as(_i64p),
+                                    orb.ctypes.data_as(C.POINTER(_u64)), cap)
+    if k < 0:
+        raise RuntimeError("bko_exhaustive_minima failed")
+    out, pos = [], 0
+    for lv in range(k):
+        cnt = int(lo[lv])
+        out.append({"E0": int(le[lv]), "orbits": [int(x) for x in orb[pos:pos + cnt]]})
+        pos += cnt
+    return {"n": n, "e_star": es.value, "total_minima": tm.value, "total_orbits": to.value,
+            "num_energies": ne.value, "levels": out}
+
+
+def ref_exhaustive_minima(n: int, with_sd: bool = False, num_sd: int = 100000, json_path=None,
+                          cap: int = 1 << 16):
+    """The reference's own run_exploration (oracle/_ref), or None when not built."""
+    R = ref()
+    if R is None:
+        return None
+    es, tm, to = _i64(), _i64(), _i64()
+    le, lo = np.zeros(2, dtype=np.int64), np.zeros(2, dtype=np.int64)
+    canon = np.zeros(cap, dtype=np.uint64)
+    dmin = np.zeros(cap, dtype=np.int32)
+    sd = np.zeros(cap)
+    k = R.bkref_exhaustive_minima(n, 1 if with_sd else 0, num_sd, C.byref(es), C.byref(tm), C.byref(to),
+                                  le.ctypes.data_as(_i64p), lo.ctypes.data_as(_i64p),
+                                  canon.ctypes.data_as(C.POINTER(_u64)),
+                                  dmin.ctypes.data_as(C.POINTER(C.c_int)), _p(sd), cap,
+                                  json_path.encode() if json_path else None)
+    out, pos = [], 0
+    for lv in range(k):
+        cnt = int(lo[lv])
+        out.append({"E0": int(le[lv]),
+                    "orbits": [{"canonical": int(canon[i]), "d_min": int(dmin[i]), "sd_hit_fraction": float(sd[i])}
+                               for i in range(pos, pos + cnt)]})
+        pos += cnt
+    return {"n": n, "e_star": es.value, "total_minima": tm.value, "total_orbits": to.value, "levels": out}

@@ -48,3 +48,44 @@ void bkref_steepest_descent(int n, uint64_t start, uint64_t* final_state, int64_
 }
 
 }  // extern "C"
+
+// The reference's exhaustive local-minima explorer (src/tools/exhaustive_minima.cpp),
+// compiled from its own source file; its command-line main is renamed so the
+// shared library exports none.
+#define main bkref_exhaustive_minima_cli_main
+#include <tools/exhaustive_minima.cpp>
+#undef main
+
+extern "C" {
+
+// run_exploration (exhaustive_minima.cpp:109-216) with the optional descent
+// sampling (run_sd_benchmarks, :218-271); the levels it keeps (at most 2)
+// are written level by level: energies, orbit counts, then per orbit its
+// canonical state, d_min and sd_hit_fraction (at most cap orbits).  Returns
+// the number of levels.  With json_path non-null the result is also saved by
+// the reference's own writer (save_to_json, :328-372).
+int bkref_exhaustive_minima(int n, int with_sd, uint64_t num_sd, int64_t* e_star, int64_t* total_minima,
+                            int64_t* total_orbits, int64_t* level_energy, int64_t* level_orbits, uint64_t* canon,
+                            int* d_min, double* sd_hit, int64_t cap, const char* json_path)
+{
+    ExhaustiveResult r = run_exploration(n, 0, with_sd != 0, static_cast<size_t>(num_sd));
+    *e_star = r.e_star;
+    *total_minima = r.total_minima_found;
+    *total_orbits = r.total_orbits_found;
+    int64_t w = 0;
+    for (size_t l = 0; l < r.levels.size(); ++l) {
+        level_energy[l] = r.levels[l].energy;
+        level_orbits[l] = static_cast<int64_t>(r.levels[l].orbits.size());
+        for (const OrbitInfo& o : r.levels[l].orbits) {
+            if (w >= cap) break;
+            canon[w] = o.canonical_state.bits;
+            d_min[w] = o.d_min;
+            sd_hit[w] = o.sd_hit_fraction;
+            ++w;
+        }
+    }
+    if (json_path) save_to_json(r, json_path);
+    return static_cast<int>(r.levels.size());
+}
+
+}  // extern "C"

@@ -13,7 +13,9 @@ reference's own data files (no computation), subsampled to keep them small:
 * trajectories.json    : P_<(t), <d_H>(t) from data_processed/probability_dynamics_n*.csv
                          (1200 Lie steps of dt=0.05, 7/4 printed digits)
 * results.json         : per-(trap, g) rows of probability_dynamics_n*_results.csv
-* exhaustive_minima.json : (N, E*, E0, canonical_id, d_min) from exhaustive_minima_n*.json
+* exhaustive_minima.json : (N, E*, E0, canonical_id, bitstring, d_min, sd_hit_fraction)
+                         from exhaustive_minima_n*.json
+* exhaustive_minima_files.json : the text of exhaustive_minima_n{20,24,28}.json
 * seed_traps.json      : seed -> (start, s0, E0, evals, steps, d_min) from the
                          reference's CPU code compiled into oracle/_ref (if built)
 """
@@ -113,7 +115,18 @@ def exhaustive_minima():
         for lvl in d["levels"]:
             for o in lvl["orbits"]:
                 out.append({"n": d["N"], "e_star": d["E_star"], "e0": lvl["E0"],
-                            "canonical_id": o["canonical_id"], "d_min": o["d_min"]})
+                            "canonical_id": o["canonical_id"], "bitstring": o["bitstring"],
+                            "d_min": o["d_min"], "sd_hit_fraction": o["sd_hit_fraction"]})
+    return out
+
+
+def exhaustive_minima_files():
+    """The explorer's output files themselves (the three smallest), as text: the
+    expected bytes of the JSON writer."""
+    out = {}
+    for n in (20, 24, 28):
+        with open(os.path.join(REF, "data_processed", f"exhaustive_minima_n{n}.json")) as f:
+            out[str(n)] = f.read()
     return out
 
 
@@ -145,6 +158,7 @@ def main():
         "trajectories.json": trajectories(),
         "results.json": results(),
         "exhaustive_minima.json": exhaustive_minima(),
+        "exhaustive_minima_files.json": exhaustive_minima_files(),
     }
     st = seed_traps()
     if st is not None:

@@ -190,3 +190,41 @@ def test_cxx_header_writer_matches_reference(bk, golden, tmp_path):
     out = tmp_path / "ts.csv"
     subprocess.run([str(exe), str(inp), str(out)], check=True, timeout=60)
     assert out.read_text().splitlines() == d["time_series_lines"]
+
+
+def _minima_result(rows):
+    lv = {}
+    for r in rows:
+        lv.setdefault(r["e0"], []).append({"canonical": int(r["canonical_id"], 16), "d_min": r["d_min"],
+                                           "sd_hit_fraction": r["sd_hit_fraction"]})
+    return {"n": rows[0]["n"], "e_star": rows[0]["e_star"],
+            "levels": [{"E0": e, "orbits": o} for e, o in lv.items()]}
+
+
+@pytest.mark.parametrize("n", [20, 24, 28])
+def test_exhaustive_json_matches_reference_file(wire, golden, tmp_path, n):
+    """save_to_json (exhaustive_minima.cpp:328-372): the values of
+    exhaustive_minima_n{N}.json written through the library reproduce the file
+    byte for byte."""
+    rows = [r for r in golden("exhaustive_minima.json") if r["n"] == n]
+    for r in rows:  # the bit string is the canonical id, most significant spin first
+        assert r["bitstring"] == format(int(r["canonical_id"], 16), f"0{n}b")
+    path = str(tmp_path / f"exhaustive_minima_n{n}.json")
+    wire.export_exhaustive_json(path, _minima_result(rows))
+    assert open(path).read() == golden("exhaustive_minima_files.json")[str(n)]
+
+
+def test_exhaustive_json_matches_reference_writer(wire, orc, tmp_path):
+    """Against the reference's own save_to_json (oracle/_ref) on a result of its
+    own run_exploration with descent sampling (levels, d_min, fractions)."""
+    ref_path = str(tmp_path / "ref.json")
+    R = orc.ref_exhaustive_minima(16, with_sd=True, num_sd=3000, json_path=ref_path)
+    if R is None:
+        pytest.skip("oracle/_ref not built")
+    res = {"n": 16, "e_star": R["e_star"], "levels": R["levels"]}
+    path = str(tmp_path / "ours.json")
+    wire.export_exhaustive_json(path, res)
+    assert open(path).read() == open(ref_path).read()
+    # no level at all: the empty list of the writer
+    wire.export_exhaustive_json(path, {"n": 3, "e_star": 1, "levels": []})
+    assert open(path).read() == '{\n  "N": 3,\n  "E_star": 1,\n  "levels": [\n  ]\n}\n'
