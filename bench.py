@@ -216,7 +216,15 @@ def _sampler_e2e_bench(bk) -> dict:
         r = bk.create_sampler(30).sample(0x38C06C95, sc)
         sec = time.perf_counter() - t0
         out[name] = {"wall_s": sec, "speedup": REF_SAMPLER_N30_S / sec, "peak_p_less": r.peak_p_less,
-                     "peak_time": r.peak_time, "p_blind": r.p_blind, "points": len(r.time_points)}
+                     "peak_time": r.peak_time, "p_blind": r.p_blind, "d_min": r.d_min,
+                     "initial_rise_rate_avg": r.initial_rise_rate_avg, "points": len(r.time_points)}
+    # the reference's row for this run (probability_dynamics_n30_results.csv)
+    with open(os.path.join(ROOT, "tests", "golden", "results.json")) as f:
+        row = [x for x in json.load(f) if x["N"] == "30" and int(x["trap_bits"], 16) == 0x38C06C95
+               and float(x["g"]) == 0.3]
+    if row:
+        out["reference_row"] = {k: row[0][k] for k in ("peak_p_less", "peak_time", "p_blind", "d_min",
+                                                       "initial_rise_rate_avg", "quantum_time_sec")}
     return out
 
 
@@ -329,8 +337,8 @@ def main() -> int:
                     help="skip the trap-supply (batched steepest descent) timing")
     ap.add_argument("--no-linalg", action="store_true",
                     help="skip the H.v / Krylov / sampling timings")
-    ap.add_argument("--sampler-e2e", action="store_true",
-                    help="also time ISampler::sample on the reference's N=30 configuration and trap")
+    ap.add_argument("--no-sampler-e2e", action="store_true",
+                    help="skip ISampler::sample on the reference's N=30 configuration and trap")
     ap.add_argument("--adam", action="store_true",
                     help="also run 50 Adam steps on the N=26 adjoint workload's fields")
     ap.add_argument("--no-fp64-check", action="store_true",
@@ -375,25 +383,32 @@ def main() -> int:
     eng.set_energy_mode(bk.BK_ENERGY_CACHE16 if args.energy == "cache16"
                         else bk.BK_ENERGY_ONTHEFLY)
 
-    # warmup (fresh start from the trap), then the timed continuation
+    import torch  # device plumbing only: synchronisation around the timed regions
+
+    def timed_run(e, precision):
+        """Warmup steps from the trap, then the timed continuation of K steps
+        (P_< / <d_H> / norm every step); no per-launch events in the timed region."""
+        e.evolve(n, s0, args.g, args.dt, args.warmup, order=bk.BK_STRANG, precision=precision, e0=e0,
+                 init=True, measure_every=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = e.evolve(n, s0, args.g, args.dt, args.steps, order=bk.BK_STRANG, precision=precision, e0=e0,
+                     init=False, measure_every=1)
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
     print(f"[bench] N={n} {args.precision} warmup={args.warmup} steps={args.steps}",
           file=sys.stderr, flush=True)
-    eng.evolve(n, s0, args.g, args.dt, args.warmup, order=bk.BK_STRANG, precision=prec,
-               init=True, measure_every=1)
-    import torch  # device plumbing only: synchronisation around the timed region
-    torch.cuda.synchronize()
-    eng.timing_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res = eng.evolve(n, s0, args.g, args.dt, args.steps, order=bk.BK_STRANG, precision=prec,
-                     init=False, measure_every=1)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    rows = eng.timing_read()
-    eng.timing_enable(False)
-    sec = t1 - t0
+    res, sec = timed_run(eng, prec)
     ms_per_step = 1e3 * sec / args.steps
     value = float((1 << n) * args.steps / sec)
+    # per-kernel device times: the same continuation once more, every launch
+    # bracketed by HIP events on the library's stream
+    eng.timing_enable(True)
+    eng.evolve(n, s0, args.g, args.dt, args.steps, order=bk.BK_STRANG, precision=prec, e0=e0,
+               init=False, measure_every=1)
+    rows = eng.timing_read()
+    eng.timing_enable(False)
 
     dom = max(rows, key=lambda r: r["total_ms"])
     avg_ms = dom["total_ms"] / dom["launches"]
@@ -405,19 +420,33 @@ def main() -> int:
                            "gbs": round(r["bytes_per_launch"] / (r["total_ms"] / r["launches"])
                                         * 1e-6, 1)} for r in rows}
 
-    # the same warmup + timed steps in complex128 (FP64): the error of the timed
-    # run's P_< and <d_H> series against it, step by step
+    # the same warmup + timed steps in complex128 (the reference's precision),
+    # timed the same way: the error of the complex64 series against it, step by
+    # step, and of the final state, 1 - |<psi64|psi128>|
     fp64 = None
     if prec == bk.BK_C64 and not args.no_fp64_check:
-        print(f"[bench] N={n} c128 reference of the same {args.warmup + args.steps} steps", file=sys.stderr,
-              flush=True)
-        ref = eng.evolve(n, s0, args.g, args.dt, args.warmup + args.steps, order=bk.BK_STRANG,
-                         precision=bk.BK_C128, init=True, measure_every=1)
-        rp, rd = np.asarray(ref["p_less"][args.warmup:]), np.asarray(ref["dh"][args.warmup:])
+        print(f"[bench] N={n} c128 of the same {args.warmup} + {args.steps} steps", file=sys.stderr, flush=True)
+        # the instrumented pass moved the complex64 state on: redo the warmup and
+        # the timed steps exactly as timed, so both states sit at step W + K
+        for k, init in ((args.warmup, True), (args.steps, False)):
+            eng.evolve(n, s0, args.g, args.dt, k, order=bk.BK_STRANG, precision=prec, e0=e0, init=init,
+                       measure_every=1)
+        e128 = bk.Engine(0)
+        try:
+            ref, sec128 = timed_run(e128, bk.BK_C128)
+            ov = eng.overlap(e128)
+        finally:
+            e128.close()
+        rp, rd = np.asarray(ref["p_less"]), np.asarray(ref["dh"])
         fp64 = {"steps_compared": int(len(rp)),
                 "p_less_max_abs_err": float(np.abs(np.asarray(res["p_less"]) - rp).max()),
                 "dh_max_abs_err": float(np.abs(np.asarray(res["dh"]) - rd).max()),
-                "norm_max_abs_dev": float(np.abs(np.asarray(res["norm"]) - 1.0).max())}
+                "norm_max_abs_dev": float(np.abs(np.asarray(res["norm"]) - 1.0).max()),
+                # 1 - |<psi64|psi128>| / (|psi64| |psi128|): the norms are the runs' own last measurements
+                "state_overlap_err": float(1.0 - abs(ov) / np.sqrt(float(res["norm"][-1]) * float(ref["norm"][-1]))),
+                "state_overlap_raw": [ov.real, ov.imag],
+                "c128_ms_per_step": 1e3 * sec128 / args.steps,
+                "c128_amp_updates_per_s": float((1 << n) * args.steps / sec128)}
 
     # read+write copies of 8 GiB on this device: the grid-stride copy, the part's
     # fastest stream (4 KB per workgroup) and a copy whose workgroups hold 128 KB,
@@ -437,7 +466,7 @@ def main() -> int:
         linalg = _linalg_bench(bk, eng, args.seed)
 
     e2e = None
-    if args.sampler_e2e:
+    if not args.no_sampler_e2e:
         print("[bench] sampler run, N=30 reference configuration", file=sys.stderr, flush=True)
         e2e = _sampler_e2e_bench(bk)
 

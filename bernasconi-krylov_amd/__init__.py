@@ -178,6 +178,7 @@ _SIGS = {
     "bk_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Instance), C.c_int, C.POINTER(BatchParams),
                                C.POINTER(BatchResult), C.POINTER(BatchStats)]),
     "bk_observables": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.POINTER(C.c_double)]),
+    "bk_state_overlap": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
     "bk_count_below": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_uint64)]),
     "bk_sample": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
                             C.POINTER(C.c_int64)]),
@@ -500,6 +501,12 @@ class Engine:
         out = np.zeros(3)
         _check(lib().bk_observables(self._h, s0, e0, _dp(out)))
         return float(out[0]), float(out[1]), float(out[2])
+
+    def overlap(self, other: "Engine") -> complex:
+        """<psi_self|psi_other> of the two held states (same n, same device)."""
+        out = np.zeros(2)
+        _check(lib().bk_state_overlap(self._h, other._h, _dp(out)))
+        return complex(out[0], out[1])
 
     def count_below(self, n: int, e0: int) -> int:
         c = C.c_uint64()

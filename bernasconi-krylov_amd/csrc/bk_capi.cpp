@@ -55,6 +55,50 @@ int check_ctx(bk_ctx* c)
     if (!c) return fail(BK_E_INVALID, "null context");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return fail(BK_E_HIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e));
+    // every entry point starts here, before it fetches a phase table
+    return trim_phase_tabs(c);
+}
+
+// Bound the phase-table cache.  Called at the start of an entry point, before
+// it fetches any table, so no table handed out during a call is freed by a
+// later fetch in the same call.  Waits for the stream: queued work may still
+// read the tables.
+int trim_phase_tabs(bk_ctx* c)
+{
+    if (c->phase_tabs.size() <= 64) return BK_OK;
+    HIPCALL(hipStreamSynchronize(c->stream));
+    for (auto& kv : c->phase_tabs) kv.second.release();
+    c->phase_tabs.clear();
+    return BK_OK;
+}
+
+// Set one tuning key on t (validated); the context keeps the keys a caller set
+// and lays them over the per-state defaults (bk_set_tuning).
+int apply_tuning_key(bk_ctx* c, bk::SweepTuning& t, const std::string& k, int value)
+{
+    if (k == "grid") {
+        if (value < 1 || value > (1 << 20)) return fail(BK_E_INVALID, "grid %d", value);
+        t.grid = value;
+        HIPCALL(c->partials.ensure(bk::partials_bytes(value)));
+    } else if (k == "row_bits") {
+        if (value < 3 || value > 8) return fail(BK_E_INVALID, "row_bits %d", value);
+        t.row_bits = value;
+    } else if (k == "page_bits") {
+        if (value < 0 || value > 32) return fail(BK_E_INVALID, "page_bits %d", value);
+        t.page_bits = value;
+    } else if (k == "prefetch_low") {
+        t.prefetch_low = value != 0;
+    } else if (k == "prefetch_high") {
+        t.prefetch_high = value != 0;
+    } else if (k == "pingpong") {
+        t.pingpong = value != 0;
+    } else if (k == "fused") {
+        t.fused = value != 0;
+    } else if (k == "wide_low") {
+        t.wide_low = value != 0;
+    } else {
+        return fail(BK_E_INVALID, "unknown tuning key '%s'", k.c_str());
+    }
     return BK_OK;
 }
 
@@ -96,10 +140,6 @@ int get_phase_tab(bk_ctx* c, int n, int precision, double dtp, const void** out)
         }
         HIPCALL(buf.ensure(h.size() * sizeof(double)));
         HIPCALL(hipMemcpy(buf.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    }
-    if (c->phase_tabs.size() > 64) {  // bound the cache
-        for (auto& kv : c->phase_tabs) kv.second.release();
-        c->phase_tabs.clear();
     }
     *out = buf.p;
     c->phase_tabs.emplace(key, buf);
@@ -173,7 +213,12 @@ int ensure_state(bk_ctx* c, int n, int precision)
     c->theta_pre = 0.0;
     ++c->state_gen;
     ++c->history_id;
-    if (!c->tuning_pinned) c->tuning = bk::default_tuning(n, precision);
+    // the defaults for this n and precision, with the keys a caller set on top
+    c->tuning = bk::default_tuning(n, precision);
+    for (const auto& kv : c->tuning_overrides) {
+        const int rc = apply_tuning_key(c, c->tuning, kv.first, kv.second);
+        if (rc) return rc;
+    }
     c->tuning.timer = c->timing ? &c->timer : nullptr;
     HIPCALL(c->partials.ensure(bk::partials_bytes(c->tuning.grid)));
     return BK_OK;
@@ -299,12 +344,15 @@ int bkc::lanczos_core(bk_ctx* c, int64_t dim, const std::function<hipError_t(con
     const int kp = std::min(k_pairs, m_act);
     for (int k = 0; k < kp; ++k) evals[k] = d[k];
     if (evecs) {
+        // every pair's coefficients uploaded once, ordered on the stream before
+        // the combines; each combine reads its own column
         ScopedBuf coef;
-        HIPCALL(coef.ensure(sizeof(double) * m_act));
+        HIPCALL(coef.ensure(sizeof(double) * m_act * static_cast<size_t>(kp)));
+        HIPCALL(hipMemcpyAsync(coef.p, z.data(), sizeof(double) * m_act * static_cast<size_t>(kp),
+                               hipMemcpyHostToDevice, c->stream));
         for (int k = 0; k < kp; ++k) {
-            HIPCALL(hipMemcpy(coef.p, z.data() + static_cast<size_t>(k) * m_act, sizeof(double) * m_act,
-                              hipMemcpyHostToDevice));
-            HIPCALL(bk::launch_combine(dim, 0, m_act, dV, coef.as<double>(), dw, c->stream));
+            HIPCALL(bk::launch_combine(dim, 0, m_act, dV, coef.as<double>() + static_cast<size_t>(k) * m_act, dw,
+                                       c->stream));
             HIPCALL(hipMemcpyAsync(evecs + static_cast<size_t>(k) * dim, dw, sizeof(double) * dim,
                                    hipMemcpyDeviceToHost, c->stream));
         }
@@ -519,6 +567,41 @@ int bk_observables(bk_ctx* c, uint64_t s0, int64_t e0, double* out3)
     return bkc::guard_abi([&] { return bk_observables_impl(c, s0, e0, out3); });
 }
 
+static int bk_state_overlap_impl(bk_ctx* a, bk_ctx* b, double* out2)
+{
+    int rc = check_ctx(b);
+    if (rc) return rc;
+    if ((rc = check_ctx(a))) return rc;
+    if (!out2) return fail(BK_E_INVALID, "null output");
+    if (!a->has_state || !b->has_state) return fail(BK_E_NOSTATE, "no statevector held");
+    if (a->device != b->device || a->n != b->n)
+        return fail(BK_E_INVALID, "states of n=%d (device %d) and n=%d (device %d)", a->n, a->device, b->n, b->device);
+    if ((rc = flush_phase(b))) return rc;
+    HIPCALL(hipStreamSynchronize(b->stream));
+    if ((rc = flush_phase(a))) return rc;
+    const int grid = 4096;
+    ScopedBuf part;
+    HIPCALL(part.ensure(2 * sizeof(double) * grid));
+    HIPCALL(bk::launch_overlap(a->psi.p, a->precision, b->psi.p, b->precision, a->n, part.as<double>(), grid,
+                               a->stream));
+    std::vector<double> h(2 * grid);
+    HIPCALL(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    HIPCALL(hipStreamSynchronize(a->stream));
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < grid; ++i) {  // fixed order: the same bits every call
+        re += h[2 * i];
+        im += h[2 * i + 1];
+    }
+    out2[0] = re;
+    out2[1] = im;
+    return BK_OK;
+}
+
+int bk_state_overlap(bk_ctx* a, bk_ctx* b, double* out2)
+{
+    return bkc::guard_abi([&] { return bk_state_overlap_impl(a, b, out2); });
+}
+
 // Make the stored state carry the opening phase of a run's first step
 // (phi0 = dt for Lie, dt/2 for Strang).  fresh_start: the state is reset to
 // |s0> with that phase folded into its single amplitude, and the state-0
@@ -550,6 +633,12 @@ static int prepare_opening(bk_ctx* c, int order, double dt, bool fresh_start, ui
         return BK_OK;
     }
     const double need = phi0 - c->theta_pre;
+    if (need == 0.0 && slot0 && c->obs_valid && c->obs_gen == c->state_gen && c->obs_s0 == s0 && c->obs_e0 == e0) {
+        // the held state is the one the last run measured at its end (same
+        // reference state and threshold): its observables are known
+        HIPCALL(hipMemcpyAsync(slot0, c->obs, sizeof(c->obs), hipMemcpyHostToDevice, c->stream));
+        return BK_OK;
+    }
     if (need != 0.0) {
         const void* tab = nullptr;
         int rc = get_phase_tab(c, c->n, c->precision, need, &tab);
@@ -726,6 +815,13 @@ static int bk_evolve_impl(bk_ctx* c, const bk_evolve_params* p, bk_series* out)
             if (out->p_less) out->p_less[i] = h[3 * i];
             if (out->dh) out->dh[i] = h[3 * i + 1];
             if (out->norm) out->norm[i] = h[3 * i + 2];
+        }
+        if (slot > 0 && meas[static_cast<size_t>(slot - 1)] == K) {  // the held state's observables
+            std::copy(h.end() - 3, h.end(), c->obs);
+            c->obs_valid = true;
+            c->obs_gen = c->state_gen;
+            c->obs_s0 = s0;
+            c->obs_e0 = e0u;
         }
         out->count = slot;
         out->e0 = e0;
@@ -1180,7 +1276,14 @@ static int bk_sampler_run_impl(bk_ctx* c, const bk_sampler_config* cfg, uint64_t
     auto cleanup_fail = [&](int code) { return code; };
     // sampler.cuh:420-423
     r->start_energy = bk::host_energy(n, start);
-    r->d_min = bk::host_d_min(n, start, r->start_energy, -1);
+    if (n >= 20) {  // radii up to ~13 at n = 30: 10^8 and more energies, on the device
+        int dm = 0;
+        const int64_t tgt = r->start_energy;
+        if ((rc = bk_d_min_batch(c, n, &start, &tgt, 1, -1, &dm))) return cleanup_fail(rc);
+        r->d_min = dm;
+    } else {
+        r->d_min = bk::host_d_min(n, start, r->start_energy, -1);
+    }
     if ((rc = bk_state_init(c, n, precision, start))) return cleanup_fail(rc);
     uint64_t cnt = 0;
     if ((rc = bk_count_below(c, n, r->start_energy, &cnt))) return cleanup_fail(rc);
@@ -1387,31 +1490,9 @@ static int bk_set_tuning_impl(bk_ctx* c, const char* key, int value)
     if (!key) return fail(BK_E_INVALID, "null key");
     const std::string k = key;
     HIPCALL(hipStreamSynchronize(c->stream));
-    if (k == "grid") {
-        if (value < 1 || value > (1 << 20)) return fail(BK_E_INVALID, "grid %d", value);
-        c->tuning.grid = value;
-        HIPCALL(c->partials.ensure(bk::partials_bytes(value)));
-    } else if (k == "row_bits") {
-        if (value < 3 || value > 8) return fail(BK_E_INVALID, "row_bits %d", value);
-        c->tuning.row_bits = value;
-    } else if (k == "page_bits") {
-        if (value < 0 || value > 32) return fail(BK_E_INVALID, "page_bits %d", value);
-        c->tuning.page_bits = value;
-    } else if (k == "prefetch_low") {
-        c->tuning.prefetch_low = value != 0;
-    } else if (k == "prefetch_high") {
-        c->tuning.prefetch_high = value != 0;
-    } else if (k == "pingpong") {
-        c->tuning.pingpong = value != 0;
-        if (!value) c->psi_alt.release();
-    } else if (k == "fused") {
-        c->tuning.fused = value != 0;
-    } else if (k == "wide_low") {
-        c->tuning.wide_low = value != 0;
-    } else {
-        return fail(BK_E_INVALID, "unknown tuning key '%s'", key);
-    }
-    c->tuning_pinned = true;
+    if ((rc = apply_tuning_key(c, c->tuning, k, value))) return rc;
+    if (k == "pingpong" && !value) c->psi_alt.release();
+    c->tuning_overrides[k] = value;
     return BK_OK;
 }
 

@@ -172,7 +172,14 @@ struct bk_ctx {
     std::map<std::pair<int, std::pair<int, double>>, bkc::DevBuf> phase_tabs;  // (n, (prec, dtp))
     bk::SweepTuning tuning{};
     int energy_mode = BK_ENERGY_CACHE16;  // how the LOW sweep gets E(s)
-    bool tuning_pinned = false;           // set by bk_set_tuning: keep across states
+    std::map<std::string, int> tuning_overrides;  // keys set by bk_set_tuning, laid over each state's defaults
+    // P_<, <d_H>, norm of the held state as the last bk_evolve measured it at
+    // its end (valid while state_gen stays obs_gen), for reference s0 / threshold e0
+    double obs[3] = {0.0, 0.0, 0.0};
+    bool obs_valid = false;
+    uint64_t obs_gen = 0;
+    uint64_t obs_s0 = 0;
+    uint32_t obs_e0 = 0;
     bkc::DevBuf ecache;                        // uint16 E(s) per amplitude (LOW tile layout)
     int ecache_n = 0;
     int ecache_prec = 0;
@@ -225,6 +232,10 @@ struct bk_ctx {
 namespace bkc {
 
 int check_ctx(bk_ctx* c);
+// bound the phase-table cache; call before an entry point fetches tables
+int trim_phase_tabs(bk_ctx* c);
+// one bk_set_tuning key applied to t (validated; "grid" also sizes the partials)
+int apply_tuning_key(bk_ctx* c, bk::SweepTuning& t, const std::string& k, int value);
 bool is_fin(double x);
 // exp(-i dtp E/n) for E = 4m + c_n, m = 0..count-1, in FP64 (re, im pairs)
 std::vector<double> phase_master(int n, double dtp);

@@ -243,6 +243,11 @@ hipError_t launch_sample(const void* psi, int n, int precision, const double* d_
                          hipStream_t stream);
 size_t sample_scratch_doubles(int n);
 
+// <a|b> = sum_s conj(a_s) b_s over two 2^n states of any precisions, in FP64:
+// partials[2 i], partials[2 i + 1] = re, im of workgroup i's contiguous chunk
+hipError_t launch_overlap(const void* a, int prec_a, const void* b, int prec_b, int n, double* partials, int grid,
+                          hipStream_t stream);
+
 // ---- H.v (matrix-free, diagonalizer.cuh:60-77) ----
 // y = (E/n or E) * x - g * sum_j X_j x ; real FP64 (complex=0) or complex128 (complex=1).
 // ecache: E(s) as uint16 in state order (launch_energy_natural), required from

@@ -190,6 +190,62 @@ hipError_t launch_sample(const void* psi, int n, int precision, const double* d_
 }
 
 // ---------------------------------------------------------------------------
+// Overlap of two states
+// ---------------------------------------------------------------------------
+template <typename VA, typename VB>
+__global__ __launch_bounds__(kThreads) void k_overlap(const VA* __restrict__ a, const VB* __restrict__ b, uint64_t dim,
+                                                      double* __restrict__ partials)
+{
+    __shared__ double sc[2][kThreads / 64];
+    const uint64_t chunk = (dim + gridDim.x - 1) / gridDim.x;
+    const uint64_t beg = static_cast<uint64_t>(blockIdx.x) * chunk;
+    const uint64_t end = min(dim, beg + chunk);
+    double re = 0.0, im = 0.0;
+    for (uint64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        const double ar = static_cast<double>(a[i].x), ai = static_cast<double>(a[i].y);
+        const double br = static_cast<double>(b[i].x), bi = static_cast<double>(b[i].y);
+        re = fma(ar, br, fma(ai, bi, re));
+        im = fma(ar, bi, fma(-ai, br, im));
+    }
+    re = wsum(re);
+    im = wsum(im);
+    if ((threadIdx.x & 63) == 0) {
+        sc[0][threadIdx.x >> 6] = re;
+        sc[1][threadIdx.x >> 6] = im;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = (sc[0][0] + sc[0][1]) + (sc[0][2] + sc[0][3]);
+        partials[2 * blockIdx.x + 1] = (sc[1][0] + sc[1][1]) + (sc[1][2] + sc[1][3]);
+    }
+}
+
+template <typename VA>
+static hipError_t overlap_b(const VA* a, const void* b, int prec_b, uint64_t dim, double* partials, int grid,
+                            hipStream_t st)
+{
+    if (prec_b == kC64)
+        hipLaunchKernelGGL((k_overlap<VA, float2>), dim3(grid), dim3(kThreads), 0, st, a,
+                           static_cast<const float2*>(b), dim, partials);
+    else if (prec_b == kC128)
+        hipLaunchKernelGGL((k_overlap<VA, double2>), dim3(grid), dim3(kThreads), 0, st, a,
+                           static_cast<const double2*>(b), dim, partials);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_overlap(const void* a, int prec_a, const void* b, int prec_b, int n, double* partials, int grid,
+                          hipStream_t stream)
+{
+    static_assert(kThreads == 256, "four waves per workgroup");
+    const uint64_t dim = uint64_t(1) << n;
+    if (prec_a == kC64) return overlap_b(static_cast<const float2*>(a), b, prec_b, dim, partials, grid, stream);
+    if (prec_a == kC128) return overlap_b(static_cast<const double2*>(a), b, prec_b, dim, partials, grid, stream);
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
 // H.v
 // ---------------------------------------------------------------------------
 struct Dbl1 {

@@ -267,6 +267,11 @@ int bk_batch_run(bk_ctx* ctx, const bk_instance* inst, int count, const bk_batch
 
 /* P_<, <d_H>, norm of the held state (out[3]) */
 int bk_observables(bk_ctx* ctx, uint64_t s0, int64_t e0, double* out3);
+/* <psi_a|psi_b> = sum_s conj(a_s) b_s (out2 = re, im) of the states two
+ * contexts on one device hold (same n, either precision), in FP64 with a
+ * fixed summation order.  No counterpart in the reference; the state-overlap
+ * error 1 - |<psi64|psi128>| of a complex64 run against complex128. */
+int bk_state_overlap(bk_ctx* a, bk_ctx* b, double* out2);
 /* exact #{ s : E(s) < e0 } over 2^n states, on the device */
 int bk_count_below(bk_ctx* ctx, int n, int64_t e0, uint64_t* count);
 
@@ -421,7 +426,8 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  * "wide_low" : 1 (default) = a complex64 fused run whose two high groups would
  *              need 16-amplitude rows (n = 32) takes a 14-qubit LOW tile instead,
  *              so both groups keep 32-amplitude rows; 0 = the 13-qubit tile
- * Values persist for the context until changed. */
+ * A key keeps its value for the context until set again; every other key takes
+ * the defaults of each new state's n and precision. */
 int bk_set_tuning(bk_ctx* ctx, const char* key, int value);
 
 /* The HIGH sweeps one Trotter step runs for (n, precision, row_bits, page_bits):

@@ -31,7 +31,7 @@ def _serial(bk, engine, inst, dt, steps, order, prec):
                                      ("c64", [24, 25, 24]),    # two high-qubit groups
                                      ("c128", [22, 23])])
 @pytest.mark.parametrize("order", ["strang", "lie"])
-def test_batch_bit_identical_to_serial(bk, engine, prec, ns, order):
+def test_batch_bit_identical_to_serial(bk, engine, orc, prec, ns, order):
     precision = bk.BK_C64 if prec == "c64" else bk.BK_C128
     o = bk.BK_STRANG if order == "strang" else bk.BK_LIE
     gs = [0.1 + 0.07 * i for i in range(len(ns))]
@@ -41,6 +41,13 @@ def test_batch_bit_identical_to_serial(bk, engine, prec, ns, order):
         got, st = engine.batch_run(inst, 0.02, 37, order=o, precision=precision, use_graph=graph)
         assert np.array_equal(got, ref), (graph, np.abs(got - ref).max())
         assert st["launches"] > 0 and st["device_ms"] > 0
+    # and one instance of the mix against the FP64 oracle
+    i = int(np.argmin(ns))
+    n, s0, e0, g = inst[i]
+    fp = orc.trotter(n, s0, g, 0.02, 37, order=2 if o == bk.BK_STRANG else 1, e0=e0)
+    tol = (1e-12, 1e-11) if prec == "c128" else (1e-5, 1e-4)
+    assert abs(got[i, 0] - fp["p_less"][-1]) <= tol[0]
+    assert abs(got[i, 1] - fp["dh"][-1]) <= tol[1]
 
 
 def test_batch_mixed_butterfly_forms(bk, engine):
@@ -97,3 +104,46 @@ def test_batch_rejects_small_n(bk, engine):
         engine.batch_run([(10, 0, -1, 0.3)], 0.02, 5)
     assert ei.value.code == -1
 
+
+
+@pytest.mark.slow
+def test_batched_workload_512_traps(bk, engine, orc):
+    """The batched workload at its stated size: 512 traps, n drawn from
+    {16, 18, 20, 22} by mt19937_64(1), g_i = 0.10 + 0.50 (i mod 16) / 15, Strang
+    dt = 0.02 x 500 steps, complex64, one autotuned hipGraph launch.  Checks:
+    4 instances of every n bit-identical to serial bk_evolve runs; 3 instances
+    each of n = 16 and 18 against the FP64 oracle (P_< to 1e-5, <d_H> to 1e-4);
+    every n = 20 / 22 instance against a complex128 batched run of the same
+    instances (P_< to 1e-5, <d_H> to 1e-4)."""
+    dt, steps = 0.02, 500
+    work = bk.batch_workload(512, seed=1)
+    print("batched workload: 512 traps, 500 steps, graph + autotune", flush=True)
+    got, st = engine.batch_run(work, dt, steps, order=bk.BK_STRANG, precision=bk.BK_C64, use_graph=True,
+                               autotune=True)
+    assert st["tuned"] and got.shape == (512, 3)
+    assert np.all(np.abs(got[:, 2] - 1.0) <= 1e-5)
+    by_n = {}
+    for i, (n, _, _, _) in enumerate(work):
+        by_n.setdefault(n, []).append(i)
+    assert sorted(by_n) == [16, 18, 20, 22]
+    for n, idx in sorted(by_n.items()):
+        print(f"  n={n}: {len(idx)} traps; serial check of 4", flush=True)
+        for i in idx[:4]:
+            _, s0, e0, g = work[i]
+            r = engine.evolve(n, s0, g, dt, steps, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0, measure_every=0)
+            assert np.array_equal(got[i], [r["p_less"][-1], r["dh"][-1], r["norm"][-1]]), (n, i)
+    for n in (16, 18):
+        for i in by_n[n][4:7]:
+            _, s0, e0, g = work[i]
+            print(f"  n={n} instance {i}: FP64 oracle", flush=True)
+            fp = orc.trotter(n, s0, g, dt, steps, order=2, e0=e0)
+            assert abs(got[i, 0] - fp["p_less"][-1]) <= 1e-5, (n, i)
+            assert abs(got[i, 1] - fp["dh"][-1]) <= 1e-4, (n, i)
+    big = by_n[20] + by_n[22]
+    print(f"  {len(big)} n = 20 / 22 traps in complex128", flush=True)
+    ref, _ = engine.batch_run([work[i] for i in big], dt, steps, order=bk.BK_STRANG, precision=bk.BK_C128,
+                              use_graph=True)
+    assert np.abs(got[big, 0] - ref[:, 0]).max() <= 1e-5
+    assert np.abs(got[big, 1] - ref[:, 1]).max() <= 1e-4
+    print(f"  max|dP_<| {np.abs(got[big, 0] - ref[:, 0]).max():.2e}, "
+          f"max|d<d_H>| {np.abs(got[big, 1] - ref[:, 1]).max():.2e}", flush=True)

@@ -5,14 +5,14 @@
   error shrinking as dt^2 (P_< and <d_H> are diagonal, where Lie and Strang agree);
 * N=24, d_min >= 6 trap: 1000 Strang steps in complex64 against complex128
   (P_<, <d_H> every 10 steps; state overlap every 100);
-* N=32 (34.4 GB complex64, 68.7 GB complex128): 20 Strang steps in complex64
+* N=32 (34.4 GB complex64, 68.7 GB complex128): 500 Strang steps in complex64
   against complex128 and the norm, on the largest state the engine takes;
 * N=26: P_<(T) of Strang dt=0.02 against the FP64 matrix-free Krylov
   propagator (m = 30).  At N=26 the spectrum of H = E/N - g sum X spans ~220,
   and restarting every 0.5 leaves the m=30 Krylov step unconverged (measured:
   P_<(40) = 0.011593 against 0.012058 from restarts every 0.25, which agrees
   with the dt -> 0 Richardson limit of the Trotter runs to 1e-10); the test
-  restarts every 0.25 and stops at T = 10 to stay within a minute.
+  restarts every 0.25 up to T = 40 (160 restarts).
 """
 from __future__ import annotations
 
@@ -96,21 +96,24 @@ def test_n30_c64_against_c128(bk):
 
 @pytest.mark.slow
 def test_n32_c64_against_c128(bk):
+    """N=32, 500 Strang steps, complex64 against complex128 every 10 steps:
+    P_< within 1e-6; <d_H> (about 14 here) within 1e-4, its error following
+    the complex64 norm drift (1e-6 after 500 steps: 1.4e-5 measured)."""
     n = 32
     _, s0, e0 = bk.make_trap(n, 1)
     out = {}
     for prec in (bk.BK_C64, bk.BK_C128):
         eng = bk.Engine(0)
         try:
-            out[prec] = eng.evolve(n, s0, 0.35, 0.02, 20, order=bk.BK_STRANG, precision=prec, e0=e0,
-                                   measure_every=5)
+            out[prec] = eng.evolve(n, s0, 0.35, 0.02, 500, order=bk.BK_STRANG, precision=prec, e0=e0,
+                                   measure_every=10)
         finally:
             eng.close()
     a, b = out[bk.BK_C64], out[bk.BK_C128]
     print(f"\n[N=32] P_< c64 {a['p_less'][-1]:.10e} c128 {b['p_less'][-1]:.10e}; "
           f"norm c64 {a['norm'][-1]:.12f}")
     assert np.abs(a["p_less"] - b["p_less"]).max() <= 1e-6
-    assert np.abs(a["dh"] - b["dh"]).max() <= 1e-5
+    assert np.abs(a["dh"] - b["dh"]).max() <= 1e-4
     assert np.abs(b["norm"] - 1.0).max() <= 1e-12
     assert np.abs(a["norm"] - 1.0).max() <= 1e-5
 
@@ -141,7 +144,10 @@ def test_n32_wide_low_tile_matches_narrow_plan(bk):
 
 @pytest.mark.slow
 def test_n26_trotter_against_krylov(bk, engine):
-    n, g, T = 26, 0.35, 10.0
+    """P_<(T = 40) at N=26: Strang dt = 0.02 (2000 steps) against the FP64
+    Krylov propagator (m = 30, restarted every 0.25: at 0.5 the m = 30 step is
+    not converged, since the spectrum of H spans about 220)."""
+    n, g, T = 26, 0.35, 40.0
     _, s0, e0 = bk.make_trap(n, 1)
     kr = bk.Engine(0)
     try:

@@ -215,3 +215,24 @@ def test_fused_run_matches_step_by_step(bk, n):
     c = out[(bk.BK_C64, 1)][0]
     assert np.abs(c["p_less"] - a["p_less"]).max() <= 1e-6
     assert np.abs(c["dh"] - a["dh"]).max() <= 1e-5
+
+
+def test_tuning_key_keeps_other_defaults(bk):
+    """bk_set_tuning pins only the key it sets: a later state of another n or
+    precision takes that state's defaults for every other key (row bits,
+    planner page granule), so setting 'grid' to its default value changes
+    nothing, bit for bit, against a context that never set a key."""
+    n = 24
+    _, s0, e0 = bk.make_trap(n, 2)
+    out = []
+    for pin in (False, True):
+        eng = bk.Engine(0)
+        try:
+            if pin:
+                eng.set_tuning("grid", 4096)
+                eng.evolve(16, 1, 0.3, 0.02, 3, precision=bk.BK_C64)  # another state first
+            r = eng.evolve(n, s0, 0.35, 0.02, 30, order=bk.BK_STRANG, precision=bk.BK_C128, e0=e0, measure_every=5)
+            out.append(np.concatenate([r["p_less"], r["dh"], r["norm"]]))
+        finally:
+            eng.close()
+    assert np.array_equal(out[0], out[1])
