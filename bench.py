@@ -285,6 +285,45 @@ def _batch_bench(bk, eng, seed: int) -> dict:
     }
 
 
+def _minima_bench(bk, eng) -> dict:
+    """The exhaustive local-minima explorer at N=30 (2^29 half-space states):
+    scan, the two trap levels, exact d_min of every orbit and 10^5 seed-42
+    descents for the hit fractions, checked against the reference's
+    exhaustive_minima_n30.json (tests/golden); beside it the reference's own
+    run_exploration (oracle/_ref, one host thread) at N=24 when it was built."""
+    n = 30
+    eng.exhaustive_minima(20)  # warm
+    t0 = time.perf_counter()
+    r = eng.exhaustive_minima(n, num_sd_starts=100000, sd_seed=42)
+    wall = time.perf_counter() - t0
+    with open(os.path.join(ROOT, "tests", "golden", "exhaustive_minima.json")) as f:
+        rows = [x for x in json.load(f) if x["n"] == n]
+    got = [(lv["E0"], o["canonical"], o["d_min"]) for lv in r["levels"] for o in lv["orbits"]]
+    want = [(x["e0"], int(x["canonical_id"], 16), x["d_min"]) for x in rows]
+    out = {"workload": f"N={n}: 2^{n - 1} states scanned for one-flip minima, 2 trap levels with exact d_min, "
+                       f"100000 seed-42 descents",
+           "scan_s": r["search_time_sec"], "total_s": wall,
+           "states_per_s": float(1 << (n - 1)) / r["search_time_sec"],
+           "minima": r["total_minima_found"], "orbits": r["total_orbits_found"],
+           "matches_reference_file": r["e_star"] == rows[0]["e_star"] and got == want}
+    so = os.path.join(ROOT, "oracle", "_ref", "libbk_ref.so")
+    if os.path.exists(so):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the reference's compiled explorer, timed as the CPU baseline of this leg
+        m = 24
+        t0 = time.perf_counter()
+        R = O.ref_exhaustive_minima(m)
+        t_ref = time.perf_counter() - t0
+        g0 = time.perf_counter()
+        G = eng.exhaustive_minima(m)
+        t_gpu = time.perf_counter() - g0
+        out["reference_cpu"] = {"n": m, "seconds": t_ref, "states_per_s": float(1 << (m - 1)) / t_ref, "cores": 1,
+                                "kind": "reference", "gpu_seconds_same_n": t_gpu,
+                                "same_result": (G["e_star"], G["total_minima_found"], G["total_orbits_found"]) ==
+                                               (R["e_star"], R["total_minima"], R["total_orbits"])}
+    return out
+
+
 def _trap_supply_bench(bk, eng, seed: int) -> dict:
     """Trap supply: steepest descent of 2^18 random N=30 starts on the GPU
     (bk_descent_batch), beside the oracle's descent (one host thread) on the
@@ -335,6 +374,8 @@ def main() -> int:
                     help="skip the 512-trap batched timing")
     ap.add_argument("--no-traps", action="store_true",
                     help="skip the trap-supply (batched steepest descent) timing")
+    ap.add_argument("--no-minima", action="store_true",
+                    help="skip the exhaustive local-minima explorer at N=30")
     ap.add_argument("--no-linalg", action="store_true",
                     help="skip the H.v / Krylov / sampling timings")
     ap.add_argument("--no-sampler-e2e", action="store_true",
@@ -485,6 +526,11 @@ def main() -> int:
         print("[bench] trap supply N=30", file=sys.stderr, flush=True)
         traps = _trap_supply_bench(bk, eng, args.seed)
 
+    minima = None
+    if not args.no_minima:
+        print("[bench] exhaustive local minima N=30", file=sys.stderr, flush=True)
+        minima = _minima_bench(bk, eng)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -548,6 +594,7 @@ def main() -> int:
         "linalg": linalg,
         "batched": batch,
         "trap_supply": traps,
+        "exhaustive_minima": minima,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

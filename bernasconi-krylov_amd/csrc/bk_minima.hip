@@ -73,17 +73,50 @@ __device__ __forceinline__ bool one_flip_minimum(uint32_t s, int n, int e)
     return true;
 }
 
-// pass 0: hist_min[E] += minima, hist_rep[E] += orbit representatives
+// pass 0: hist_min[E] += minima, hist_rep[E] += orbit representatives.
+// A state's test stops at its first lower neighbour, after a few energies for
+// most states, but a wave would run as long as its slowest lane.  So every lane
+// computes one energy per iteration -- E(s) of a state it has just taken, or
+// E(s ^ 2^p) of the next flip of the state it holds -- and a lane whose state
+// is settled takes the next one of its wave's chunk.  The lanes stay busy and
+// the energies computed per state are the ones the test needs.
+constexpr uint64_t kScanChunk = 1u << 14;  // states per wave and chunk
 __global__ __launch_bounds__(kScanThreads) void k_minima_hist(int n, uint64_t half, uint32_t* __restrict__ hist_min,
                                                               uint32_t* __restrict__ hist_rep)
 {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kScanThreads;
-    for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * kScanThreads + threadIdx.x; s < half; s += stride) {
-        const uint32_t b = static_cast<uint32_t>(s);
-        const int e = energy32(b, n);
-        if (!one_flip_minimum(b, n, e)) continue;
-        atomicAdd(&hist_min[e], 1u);
-        if (canonical32(b, n) == b) atomicAdd(&hist_rep[e], 1u);
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const uint64_t below = (uint64_t(1) << lane) - 1u;
+    const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kScanThreads / 64);
+    const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (kScanThreads / 64) + (threadIdx.x >> 6);
+    for (uint64_t c0 = wave * kScanChunk; c0 < half; c0 += nwaves * kScanChunk) {
+        const uint64_t c1 = min(half, c0 + kScanChunk);
+        uint64_t next = c0;  // the chunk's next untaken state (the same in every lane)
+        uint32_t s = 0;
+        int e = 0;
+        int p = -1;  // -1: needs a state; 0..n-1: next flip to test; -2: chunk exhausted
+        for (;;) {
+            const bool need = p == -1;
+            const uint64_t want = __ballot(need);
+            const uint64_t idx = next + static_cast<uint64_t>(__popcll(want & below));
+            next += static_cast<uint64_t>(__popcll(want));
+            if (need) {
+                if (idx < c1) s = static_cast<uint32_t>(idx);
+                else p = -2;
+            }
+            if (__ballot(p != -2) == 0) break;
+            if (p == -2) continue;
+            const int E = energy32(s ^ (need ? 0u : 1u << (p & 31)), n);
+            if (need) {
+                e = E;
+                p = 0;
+            } else if (E < e) {
+                p = -1;  // a lower neighbour: not a minimum
+            } else if (++p == n) {
+                atomicAdd(&hist_min[e], 1u);
+                if (canonical32(s, n) == s) atomicAdd(&hist_rep[e], 1u);
+                p = -1;
+            }
+        }
     }
 }
 
@@ -286,8 +319,10 @@ static int bk_exhaustive_minima_impl(bk_ctx* c, int n, int levels, uint64_t num_
     HIPCALL(hipEventRecord(e0.e, c->stream));
     uint32_t* hmin = hist.as<uint32_t>();
     uint32_t* hrep = hmin + (emax + 1);
-    hipLaunchKernelGGL(bk::k_minima_hist, dim3(bk::grid_for(half)), dim3(bk::kScanThreads), 0, c->stream, n, half,
-                       hmin, hrep);
+    const uint64_t chunks = (half + bk::kScanChunk - 1) / bk::kScanChunk;
+    const int hist_grid = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>((chunks + 3) / 4, 8192)));
+    hipLaunchKernelGGL(bk::k_minima_hist, dim3(hist_grid), dim3(bk::kScanThreads), 0, c->stream, n, half, hmin,
+                       hrep);
     HIPCALL(hipGetLastError());
     std::vector<uint32_t> h(2 * (emax + 1));
     HIPCALL(hipMemcpyAsync(h.data(), hist.p, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, c->stream));
