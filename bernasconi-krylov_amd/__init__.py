@@ -156,6 +156,8 @@ _SIGS = {
     "bk_last_error": (C.c_char_p, []),
     "bk_version": (C.c_int, []),
     "bk_device_count": (C.c_int, []),
+    "bk_set_device_memory_limit": (C.c_int, [C.c_uint64]),
+    "bk_device_memory_held": (C.c_uint64, []),
     "bk_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "bk_ctx_destroy": (None, [C.c_void_p]),
     "bk_energy": (C.c_int64, [C.c_int, C.c_uint64]),

@@ -368,6 +368,14 @@ const char* bk_last_error(void) { return g_err.c_str(); }
 
 int bk_version(void) { return 100; }
 
+int bk_set_device_memory_limit(uint64_t bytes)
+{
+    bkc::g_dev_limit = bytes;
+    return BK_OK;
+}
+
+uint64_t bk_device_memory_held(void) { return bkc::g_dev_held.load(); }
+
 int bk_device_count(void)
 {
     int n = 0;

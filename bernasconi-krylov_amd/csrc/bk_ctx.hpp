@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <exception>
 #include <functional>
@@ -47,6 +48,12 @@ int guard_abi(F&& body)
 
 inline size_t amp_bytes(int precision) { return precision == BK_C64 ? 8 : 16; }
 
+// Device bytes the library holds in this process, and the cap a caller may set
+// on them (bk_set_device_memory_limit; 0 = none): every device allocation
+// goes through DevBuf, and one beyond the cap fails as hipErrorOutOfMemory.
+inline std::atomic<uint64_t> g_dev_held{0};
+inline std::atomic<uint64_t> g_dev_limit{0};
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -56,16 +63,25 @@ struct DevBuf {
         if (p) {
             hipError_t e = hipFree(p);
             if (e != hipSuccess) return e;
+            g_dev_held -= cap;
             p = nullptr;
             cap = 0;
         }
+        const uint64_t limit = g_dev_limit.load();
+        if (limit && g_dev_held.load() + bytes > limit) return hipErrorOutOfMemory;
         hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipSuccess) cap = bytes;
+        if (e == hipSuccess) {
+            cap = bytes;
+            g_dev_held += bytes;
+        }
         return e;
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            g_dev_held -= cap;
+        }
         p = nullptr;
         cap = 0;
     }

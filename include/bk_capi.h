@@ -50,6 +50,14 @@ int bk_version(void);
 /* Number of HIP devices (0 when none), without creating a context. */
 int bk_device_count(void);
 
+/* Device memory the library holds in this process (all contexts), and a cap on
+ * it (0 = none, the default).  An allocation that would pass the cap fails
+ * with BK_E_NOMEM, as it would on a full device, and leaves the context usable.
+ * No counterpart in the reference (its CHECK_CUDA throws on a failed cudaMalloc,
+ * sampler.cuh:31-47, 429-431); for callers that share a device. */
+int bk_set_device_memory_limit(uint64_t bytes);
+uint64_t bk_device_memory_held(void);
+
 /* One context per device (replaces custatevecCreate/Destroy, sampler.cuh:448-449, 703). */
 int bk_ctx_create(int device, bk_ctx** out);
 void bk_ctx_destroy(bk_ctx* ctx);
