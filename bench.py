@@ -289,8 +289,7 @@ def _minima_bench(bk, eng) -> dict:
     """The exhaustive local-minima explorer at N=30 (2^29 half-space states):
     scan, the two trap levels, exact d_min of every orbit and 10^5 seed-42
     descents for the hit fractions, checked against the reference's
-    exhaustive_minima_n30.json (tests/golden); beside it the reference's own
-    run_exploration (oracle/_ref, one host thread) at N=24 when it was built."""
+    exhaustive_minima_n30.json (tests/golden)."""
     n = 30
     eng.exhaustive_minima(20)  # warm
     t0 = time.perf_counter()
@@ -306,31 +305,14 @@ def _minima_bench(bk, eng) -> dict:
            "states_per_s": float(1 << (n - 1)) / r["search_time_sec"],
            "minima": r["total_minima_found"], "orbits": r["total_orbits_found"],
            "matches_reference_file": r["e_star"] == rows[0]["e_star"] and got == want}
-    so = os.path.join(ROOT, "oracle", "_ref", "libbk_ref.so")
-    if os.path.exists(so):
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O  # the reference's compiled explorer, timed as the CPU baseline of this leg
-        m = 24
-        t0 = time.perf_counter()
-        R = O.ref_exhaustive_minima(m)
-        t_ref = time.perf_counter() - t0
-        g0 = time.perf_counter()
-        G = eng.exhaustive_minima(m)
-        t_gpu = time.perf_counter() - g0
-        out["reference_cpu"] = {"n": m, "seconds": t_ref, "states_per_s": float(1 << (m - 1)) / t_ref, "cores": 1,
-                                "kind": "reference", "gpu_seconds_same_n": t_gpu,
-                                "same_result": (G["e_star"], G["total_minima_found"], G["total_orbits_found"]) ==
-                                               (R["e_star"], R["total_minima"], R["total_orbits"])}
     return out
 
 
-def _trap_supply_bench(bk, eng, seed: int) -> dict:
+def _trap_supply_bench(bk, eng, seed: int):
     """Trap supply: steepest descent of 2^18 random N=30 starts on the GPU
-    (bk_descent_batch), beside the oracle's descent (one host thread) on the
-    first 2000 of the same starts, which must agree bit for bit."""
+    (bk_descent_batch).  Returns the bench object and the first starts with
+    their GPU results, which the CPU-baseline leg descends again on the host."""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # CPU restatement: checker and timed baseline only
     n, count = 30, 1 << 18
     starts = np.random.default_rng(seed).integers(0, 1 << n, size=count, dtype=np.uint64)
     eng.descent_batch(n, starts[:4096])  # warm
@@ -340,18 +322,42 @@ def _trap_supply_bench(bk, eng, seed: int) -> dict:
     fin, fe, ev = eng.descent_batch(n, starts)
     t_gpu = time.perf_counter() - t0
     k = 2000
-    t0 = time.perf_counter()
-    ref = [O.steepest_descent(n, int(s)) for s in starts[:k]]
-    t_cpu = time.perf_counter() - t0
-    same = all((int(fin[i]), int(fe[i]), int(ev[i])) == ref[i][:3] for i in range(k))
-    return {
+    out = {
         "workload": f"steepest descent of {count} random N={n} starts (numpy default_rng seed {seed})",
         "gpu_descents_per_s": count / t_gpu,
         "gpu_energy_evals_per_s": float(ev.sum()) / t_gpu,
-        "cpu_descents_per_s": k / t_cpu,
-        "cpu_sample": f"oracle descent, 1 thread, first {k} starts",
-        "bit_identical_on_sample": same,
     }
+    return out, (n, starts[:k], fin[:k], fe[:k], ev[:k])
+
+
+def _trap_supply_cpu(O, sample) -> dict:
+    """The oracle's host descent (one thread) of the same first starts: rate,
+    and bit-identity with the GPU's final states, energies and evaluations."""
+    n, starts, fin, fe, ev = sample
+    t0 = time.perf_counter()
+    ref = [O.steepest_descent(n, int(s)) for s in starts]
+    t_cpu = time.perf_counter() - t0
+    same = all((int(fin[i]), int(fe[i]), int(ev[i])) == ref[i][:3] for i in range(len(starts)))
+    return {"value": len(starts) / t_cpu, "unit": "descents/s", "cores": 1, "kind": "port",
+            "sample": f"oracle steepest descent, 1 thread, the first {len(starts)} N={n} starts of the trap-supply leg",
+            "bit_identical_to_gpu": same}
+
+
+def _minima_cpu(O, eng) -> dict:
+    """The reference's own run_exploration (oracle/_ref, one host thread) at N=24
+    beside the GPU explorer at the same n."""
+    m = 24
+    t0 = time.perf_counter()
+    R = O.ref_exhaustive_minima(m)
+    t_ref = time.perf_counter() - t0
+    g0 = time.perf_counter()
+    G = eng.exhaustive_minima(m)
+    t_gpu = time.perf_counter() - g0
+    return {"value": float(1 << (m - 1)) / t_ref, "unit": "states/s", "cores": 1, "kind": "reference",
+            "sample": f"the reference's exhaustive-minima explorer (oracle/_ref) at N={m}: {t_ref:.2f} s",
+            "gpu_seconds_same_n": t_gpu,
+            "same_result": (G["e_star"], G["total_minima_found"], G["total_orbits_found"]) ==
+                           (R["e_star"], R["total_minima"], R["total_orbits"])}
 
 
 def main() -> int:
@@ -522,10 +528,10 @@ def main() -> int:
         print("[bench] batched 512 traps", file=sys.stderr, flush=True)
         batch = _batch_bench(bk, eng, args.seed)
 
-    traps = None
+    traps = trap_sample = None
     if not args.no_traps:
         print("[bench] trap supply N=30", file=sys.stderr, flush=True)
-        traps = _trap_supply_bench(bk, eng, args.seed)
+        traps, trap_sample = _trap_supply_bench(bk, eng, args.seed)
 
     minima = None
     if not args.no_minima:
@@ -543,6 +549,14 @@ def main() -> int:
         print(f"[bench] cpu baseline N={n1} threads=1", file=sys.stderr, flush=True)
         one = _cpu_baseline(n1, args.seed, 1, f"the N={args.cpu_n} trap construction at N={n1}")
         cpu["single_core"] = {"value": one["value"], "unit": one["unit"], "cores": 1, "sample": one["sample"]}
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # CPU restatement and the reference's compiled code: timed baselines and checkers only
+        if trap_sample is not None:
+            print("[bench] cpu baseline: trap-supply descents", file=sys.stderr, flush=True)
+            cpu["trap_supply"] = _trap_supply_cpu(O, trap_sample)
+        if minima is not None and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libbk_ref.so")):
+            print("[bench] cpu baseline: the reference's explorer N=24", file=sys.stderr, flush=True)
+            cpu["exhaustive_minima"] = _minima_cpu(O, eng)
 
     out = {
         "metric": "Trotter amp-updates/s & % HBM peak, N=30 c64; adjoint/forward time ratio",
