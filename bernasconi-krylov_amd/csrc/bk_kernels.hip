@@ -1662,7 +1662,8 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
             coef(k, &b.bc, &b.bs);
             const int g = clamp_grid(b.ntiles, tun.grid);
             if (tun.timer) tun.timer->begin(st);
-            e = launch_high_dyn<V>(tan, false, psi, A.R, b, g, st);
+            // the next tile's loads go out before this tile's rotations (N=30: 3.242 -> 3.225 ms)
+            e = launch_high_dyn<V>(tan, true, psi, A.R, b, g, st);
             if (e != hipSuccess) return e;
             if (tun.timer) tun.timer->end(st, "fused_high_a", pass_bytes);
         }
