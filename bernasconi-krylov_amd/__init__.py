@@ -523,7 +523,8 @@ class Engine:
         return out
 
     def set_energy_mode(self, mode: int):
-        """BK_ENERGY_CACHE16 (uint16 E(s) array) or BK_ENERGY_ONTHEFLY (popcounts)."""
+        """BK_ENERGY_CACHE16 (exact E(s) cache, 12-bit codes, 1.5 B per state) or
+        BK_ENERGY_ONTHEFLY (popcounts)."""
         _check(lib().bk_set_energy_mode(self._h, mode))
 
     def set_tuning(self, key: str, value: int):

@@ -48,7 +48,7 @@ struct Group {
     int tab_count = 0;     // entries per table
     size_t master_off = 0; // double offset in the master arena
     std::vector<bk::HighSweep> fused;   // high groups of a fused run (empty: step by step)
-    const uint16_t* ecache_hi = nullptr; // E(s) in the phase-carrying high group's layout
+    const uint32_t* ecache_hi = nullptr; // E(s) in the phase-carrying high group's layout
 };
 
 struct Tune {
@@ -82,7 +82,7 @@ struct Plan {
     double* partials = nullptr;
     double* results = nullptr;
     char* alt = nullptr;  // out-of-place blocks: the second buffer of a block
-    std::map<int, const uint16_t*> ecache;
+    std::map<int, const uint32_t*> ecache;
     size_t total_states_bytes = 0;
 };
 
@@ -344,10 +344,10 @@ static int bk_batch_run_impl(bk_ctx* c, const bk_instance* inst, int count, cons
     for (Group& g : P.groups) {
         DevBuf& ec = c->batch_ecache[{g.n, prec}];
         if (!ec.p) {
-            HIPCALL(ec.ensure(sizeof(uint16_t) << g.n));
-            HIPCALL(bk::launch_energy_fill(ec.as<uint16_t>(), g.n, prec, c->stream));
+            HIPCALL(ec.ensure(bk::energy_cache_bytes(g.n)));
+            HIPCALL(bk::launch_energy_fill(ec.as<uint32_t>(), g.n, prec, c->stream));
         }
-        P.ecache[g.n] = ec.as<uint16_t>();
+        P.ecache[g.n] = ec.as<uint32_t>();
         // fused schedule (default tuning: the tuner's row bits steer step-by-step sweeps only)
         std::vector<bk::HighSweep> fg;
         if (g.tan && c->tuning.fused) fg = bk::fused_sweeps(g.n, prec, bk::default_tuning(g.n, prec));
@@ -356,10 +356,10 @@ static int bk_batch_run_impl(bk_ctx* c, const bk_instance* inst, int count, cons
             const bk::HighSweep& h = fg.back();
             DevBuf& eh = c->batch_ecache_hi[{g.n, prec, h.h1, h.a1, h.h2, h.a2, h.R}];
             if (!eh.p) {
-                HIPCALL(eh.ensure(sizeof(uint16_t) << g.n));
-                HIPCALL(bk::launch_energy_fill_high(eh.as<uint16_t>(), g.n, prec, h, c->stream));
+                HIPCALL(eh.ensure(bk::energy_cache_bytes(g.n)));
+                HIPCALL(bk::launch_energy_fill_high(eh.as<uint32_t>(), g.n, prec, h, c->stream));
             }
-            g.ecache_hi = eh.as<uint16_t>();
+            g.ecache_hi = eh.as<uint32_t>();
         }
     }
 

@@ -240,15 +240,15 @@ bool is_fin(double x) { return std::isfinite(x); }
 
 }  // namespace
 
-// the uint16 E(s) cache of the held state's n, in the layout of a LOW tile of
+// the energy-code cache of the held state's n, in the layout of a LOW tile of
 // low_bits (0: the precision's default tile); refilled when n, precision or
 // the layout change
 int bkc::ensure_ecache(bk_ctx* c, int low_bits)
 {
     const int lb = low_bits > 0 ? low_bits : bk::low_tile_bits(c->precision);
     if (c->ecache_n != c->n || c->ecache_prec != c->precision || c->ecache_lb != lb) {
-        HIPCALL(c->ecache.ensure(sizeof(uint16_t) << c->n));
-        HIPCALL(bk::launch_energy_fill(c->ecache.as<uint16_t>(), c->n, c->precision, c->stream, lb));
+        HIPCALL(c->ecache.ensure(bk::energy_cache_bytes(c->n)));
+        HIPCALL(bk::launch_energy_fill(c->ecache.as<uint32_t>(), c->n, c->precision, c->stream, lb));
         c->ecache_n = c->n;
         c->ecache_prec = c->precision;
         c->ecache_lb = lb;
@@ -685,7 +685,7 @@ static int enqueue_step(bk_ctx* c, double dt, double gdt, uint64_t s0, uint32_t 
     sp.e0 = e0;
     if (c->energy_mode == BK_ENERGY_CACHE16 && c->n >= 13) {
         if ((rc = ensure_ecache(c))) return rc;
-        sp.ecache = c->ecache.as<uint16_t>();
+        sp.ecache = c->ecache.as<uint32_t>();
     }
     ++c->state_gen;
     void* alt = nullptr;
@@ -705,8 +705,8 @@ static int ensure_ecache_high(bk_ctx* c, const bk::HighSweep& hs)
     const std::vector<int> key = {c->n, c->precision, hs.h1, hs.a1, hs.h2, hs.a2, hs.R};
     if (c->ecache_hi_key != key) {
         c->ecache_hi_key.clear();
-        HIPCALL(c->ecache_hi.ensure(sizeof(uint16_t) << c->n));
-        HIPCALL(bk::launch_energy_fill_high(c->ecache_hi.as<uint16_t>(), c->n, c->precision, hs, c->stream));
+        HIPCALL(c->ecache_hi.ensure(bk::energy_cache_bytes(c->n)));
+        HIPCALL(bk::launch_energy_fill_high(c->ecache_hi.as<uint32_t>(), c->n, c->precision, hs, c->stream));
         c->ecache_hi_key = key;
     }
     return BK_OK;
@@ -766,8 +766,8 @@ static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, con
     rs.fb_count = fb->count;
     rs.s0 = static_cast<uint32_t>(s0);
     rs.e0 = e0;
-    rs.ecache = c->ecache.as<uint16_t>();
-    rs.ecache_high = c->ecache_hi.as<uint16_t>();
+    rs.ecache = c->ecache.as<uint32_t>();
+    rs.ecache_high = c->ecache_hi.as<uint32_t>();
     HIPCALL(bk::launch_run(c->psi.p, rs, c->tuning, c->partials.as<double>(), c->stream));
     ++c->state_gen;
     return BK_OK;
@@ -946,11 +946,11 @@ static int bk_adjoint_backward_impl(bk_ctx* c, double w_pless, double w_dh, doub
     rs.precision = prec;
     if (n >= low_tb) {
         if ((rc = ensure_ecache(c))) return rc;
-        rs.ecache = c->ecache.as<uint16_t>();
+        rs.ecache = c->ecache.as<uint32_t>();
         const std::vector<bk::HighSweep> grp = bk::fused_sweeps(n, prec, c->tuning);
         if (!grp.empty()) {
             if ((rc = ensure_ecache_high(c, grp.back()))) return rc;
-            rs.ecache_high = c->ecache_hi.as<uint16_t>();
+            rs.ecache_high = c->ecache_hi.as<uint32_t>();
         }
     }
     const void* tab_init = nullptr;

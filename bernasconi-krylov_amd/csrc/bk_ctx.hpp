@@ -196,7 +196,7 @@ struct bk_ctx {
     uint64_t obs_gen = 0;
     uint64_t obs_s0 = 0;
     uint32_t obs_e0 = 0;
-    bkc::DevBuf ecache;                        // uint16 E(s) per amplitude (LOW tile layout)
+    bkc::DevBuf ecache;                        // packed 12-bit codes E(s) >> 2 (LOW tile layout)
     int ecache_n = 0;
     int ecache_prec = 0;
     int ecache_lb = 0;                         // LOW tile bits of its layout

@@ -179,7 +179,7 @@ struct SweepArgs {
     const void* tab;   // phase table (V per energy) for F_PHASE
     double* partials;  // gridDim.x * 3 doubles for F_MEASURE, gridDim.x for F_GRAD
     double* partials2; // F_GRAD with F_POSTROT: gridDim.x doubles, terms of the post-phase rotations
-    const uint16_t* ecache;  // F_ECACHE: E(s) per amplitude (exact, < 2^16)
+    const uint32_t* ecache;  // F_ECACHE: packed 12-bit codes E(s) >> 2 (k_energy_fill)
     void* dst;               // output state (null: in place)
     void* psi2;              // two-state sweeps: the second state (same tiles, same rotations)
     void* dst2;              // its output (null: in place)
@@ -342,9 +342,10 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     double gacc2[1] = {0.0};
     constexpr bool PF = (FLAGS & F_PREFETCH) != 0;
     // energy cache words: the cache is stored tile by tile in the last phase's
-    // register layout (thread t's NR values contiguous), two uint16 per word
-    constexpr int NEC = (FLAGS & F_ECACHE) ? NR / 2 : 1;
-    static_assert(!(FLAGS & F_ECACHE) || (NR % 8 == 0), "cache rows are whole 16-byte words");
+    // register layout (thread t's NR codes contiguous), eight 12-bit codes per
+    // three words (k_energy_fill)
+    constexpr int NEC = (FLAGS & F_ECACHE) ? 3 * NR / 8 : 1;
+    static_assert(!(FLAGS & F_ECACHE) || (NR % 8 == 0), "cache rows are whole groups of eight codes");
 
     auto tile_base = [&](uint64_t tile) -> uint64_t {
         if constexpr (LOW) {
@@ -389,14 +390,30 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             }
         }
         if constexpr ((FLAGS & F_ECACHE) != 0) {
-            const uint4* ep = reinterpret_cast<const uint4*>(a.ecache + ecb + static_cast<uint64_t>(t) * NR);
+            // ecb counts amplitudes: 3/8 of a word each (tile bases are multiples of 8)
+            const uint32_t* ew = reinterpret_cast<const uint32_t*>(a.ecache) + (ecb / 8) * 3 +
+                                 static_cast<uint64_t>(t) * NEC;
+            if constexpr (NEC % 4 == 0) {
+                const uint4* ep = reinterpret_cast<const uint4*>(ew);
 #pragma unroll
-            for (int q = 0; q < NR / 8; ++q) {
-                const uint4 w = ep[q];
-                ev[4 * q + 0] = w.x;
-                ev[4 * q + 1] = w.y;
-                ev[4 * q + 2] = w.z;
-                ev[4 * q + 3] = w.w;
+                for (int q = 0; q < NEC / 4; ++q) {
+                    const uint4 w = ep[q];
+                    ev[4 * q + 0] = w.x;
+                    ev[4 * q + 1] = w.y;
+                    ev[4 * q + 2] = w.z;
+                    ev[4 * q + 3] = w.w;
+                }
+            } else if constexpr (NEC % 2 == 0) {
+                const uint2* ep = reinterpret_cast<const uint2*>(ew);
+#pragma unroll
+                for (int q = 0; q < NEC / 2; ++q) {
+                    const uint2 w = ep[q];
+                    ev[2 * q + 0] = w.x;
+                    ev[2 * q + 1] = w.y;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NEC; ++q) ev[q] = ew[q];
             }
         }
     };
@@ -506,6 +523,10 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                 e0v = a.e0_arr[inst];
                 s0v = a.s0_arr[inst];
             }
+            // with the cache the energy comes as its code m (E = 4m + c_n):
+            // E < e0 exactly when m < ceil((e0 - c_n) / 4)
+            [[maybe_unused]] const uint32_t cres = (static_cast<uint32_t>(a.n) >> 1) & 3u;
+            [[maybe_unused]] const uint32_t m0 = e0v > cres ? (e0v - cres + 3u) >> 2 : 0u;
             R tacc[3] = {R(0), R(0), R(0)};
             // d_H weight of register j: s = T | off_j with T = base + tgL (thread and
             // tile bits) and off_j on the register bits M, disjoint, so
@@ -526,12 +547,24 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             }
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                uint32_t E;
-                if constexpr ((FLAGS & F_ECACHE) != 0) E = (ec[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-                else E = labs_energy_u32(static_cast<uint32_t>(base + tgL + gl(reg_pattern(P.ph[NP - 1], RB, j))), a.n);
+                uint32_t mcode;  // the phase table's index, E >> 2
+                bool below;      // E(s) < e0
+                if constexpr ((FLAGS & F_ECACHE) != 0) {
+                    // j is a constant once the loop is unrolled: one shift or one
+                    // funnel shift per code
+                    const int bit = 12 * (j & 7), w = 3 * (j >> 3) + (bit >> 5), sh = bit & 31;
+                    if (sh <= 20) mcode = (ec[w] >> sh) & 0xFFFu;
+                    else mcode = ((ec[w] >> sh) | (ec[w + 1] << (32 - sh))) & 0xFFFu;
+                    below = mcode < m0;
+                } else {
+                    const uint32_t E =
+                        labs_energy_u32(static_cast<uint32_t>(base + tgL + gl(reg_pattern(P.ph[NP - 1], RB, j))), a.n);
+                    mcode = E >> 2;
+                    below = E < e0v;
+                }
                 if constexpr ((FLAGS & F_PHASE) != 0) {
                     // E = 4m + c_n: one entry per m; the table carries the step's scale
-                    const V ph = LDS_TAB ? ltab[E >> 2] : tab[E >> 2];
+                    const V ph = LDS_TAB ? ltab[mcode] : tab[mcode];
                     const R pr = ph.x, pi = ph.y;
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
@@ -545,7 +578,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                     // NR terms), folded into FP64 once per tile
                     const R pr = fma(x[0][j].x, x[0][j].x, x[0][j].y * x[0][j].y);
                     tacc[2] += pr;
-                    tacc[0] += (E < e0v) ? pr : R(0);
+                    tacc[0] += below ? pr : R(0);
                     tacc[1] = fma(pr, static_cast<R>(pcT + __popc(static_cast<uint32_t>(j) ^ dj)), tacc[1]);
                 }
             }
@@ -768,53 +801,76 @@ __global__ void k_set_basis(V* psi, uint64_t s0, double re, double im)
     psi[s0] = v;
 }
 
-// E(s) for every s as uint16 (E <= sum_{j<n} j^2 = 10416 at n = 32), stored in
-// the LOW sweep's order: tile by tile, thread t's NR values of its last-phase
-// registers contiguous, so the sweep reads them with 16-byte loads.
+// The energy caches hold m = E(s) >> 2 for every s: every E(s) is 4m + c_n with
+// c_n = floor(n/2) mod 4 (C_k = (n-k) mod 2), and m <= 2604 at n = 32 fits 12
+// bits.  Eight codes share three 32-bit words (1.5 B per amplitude):
+//   w0 = m0 | m1 << 12 | m2 << 24, w1 = m2 >> 8 | m3 << 4 | m4 << 16 | m5 << 28,
+//   w2 = m5 >> 4 | m6 << 8 | m7 << 20.
+// A cache is laid out tile by tile, thread t's NR codes of its last-phase
+// registers contiguous (3 NR / 8 words), so a sweep reads them with wide loads.
+__device__ __forceinline__ void pack8(uint32_t* __restrict__ w, const uint32_t (&m)[8])
+{
+    w[0] = m[0] | (m[1] << 12) | (m[2] << 24);
+    w[1] = (m[2] >> 8) | (m[3] << 4) | (m[4] << 16) | (m[5] << 28);
+    w[2] = (m[5] >> 4) | (m[6] << 8) | (m[7] << 20);
+}
+
+// LOW-sweep layout (plan PH)
 template <class PH>
-__global__ __launch_bounds__(kThreads) void k_energy_fill(uint16_t* __restrict__ out, int n, uint64_t dim)
+__global__ __launch_bounds__(kThreads) void k_energy_fill(uint32_t* __restrict__ out, int n, uint64_t dim)
 {
     constexpr Plan P = PH::plan;
     constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t tile = i >> TB;
-        const uint32_t t = static_cast<uint32_t>(i >> RB) & ((1u << (TB - RB)) - 1u);
-        const uint32_t j = static_cast<uint32_t>(i) & (NR - 1u);
-        uint32_t u = 0;
-        for (int b = 0; b < TB - RB; ++b) u |= ((t >> b) & 1u) << P.ph[NP - 1].thr[b];
-        for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
-        const uint64_t s = (tile << TB) | u;
-        out[i] = static_cast<uint16_t>(labs_energy_u32(static_cast<uint32_t>(s), n));
+    static_assert(NR % 8 == 0, "codes are packed eight at a time");
+    for (uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; g < dim / 8;
+         g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        uint32_t m[8];
+        for (int q = 0; q < 8; ++q) {
+            const uint64_t i = 8 * g + q;
+            const uint64_t tile = i >> TB;
+            const uint32_t t = static_cast<uint32_t>(i >> RB) & ((1u << (TB - RB)) - 1u);
+            const uint32_t j = static_cast<uint32_t>(i) & (NR - 1u);
+            uint32_t u = 0;
+            for (int b = 0; b < TB - RB; ++b) u |= ((t >> b) & 1u) << P.ph[NP - 1].thr[b];
+            for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
+            m[q] = labs_energy_u32(static_cast<uint32_t>((tile << TB) | u), n) >> 2;
+        }
+        pack8(out + 3 * g, m);
     }
 }
 
-// E(s) in the layout in which a HIGH pass (plan PH, geometry h/a1/h2) holds its
-// amplitudes at the phase: tile by tile, thread t's NR values of its last-phase
-// registers contiguous -- the same addressing as k_sweep's tile_base and gl.
+// the layout in which a HIGH pass (plan PH, geometry h/a1/h2) holds its
+// amplitudes at the phase: the same addressing as k_sweep's tile_base and gl
 template <class PH>
-__global__ __launch_bounds__(kThreads) void k_energy_fill_high(uint16_t* __restrict__ out, int n, int h, int a1, int h2,
+__global__ __launch_bounds__(kThreads) void k_energy_fill_high(uint32_t* __restrict__ out, int n, int h, int a1, int h2,
                                                               int lo_bits, int mid_bits, uint64_t dim)
 {
     constexpr Plan P = PH::plan;
     constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R, TT = P.TT();
     constexpr uint32_t ROWMASK = (1u << RR) - 1u;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < dim;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t tile = i >> TB;
-        const uint32_t rem = static_cast<uint32_t>(i) & ((1u << TB) - 1u);
-        const uint32_t t = rem >> RB, j = rem & (NR - 1u);
-        uint32_t u = 0;
-        for (int b = 0; b < TT; ++b) u |= ((t >> b) & 1u) << P.ph[NP - 1].thr[b];
-        for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
-        const uint32_t ua = u >> RR;
-        const uint64_t off = static_cast<uint64_t>(u & ROWMASK) + (static_cast<uint64_t>(ua & ((1u << a1) - 1u)) << h) +
-                             (static_cast<uint64_t>(ua >> a1) << h2);
-        const uint64_t lo = tile & ((1ull << lo_bits) - 1ull);
-        const uint64_t mid = (tile >> lo_bits) & ((1ull << mid_bits) - 1ull);
-        const uint64_t hi = tile >> (lo_bits + mid_bits);
-        const uint64_t base = (lo << RR) | (mid << (h + a1)) | (hi << (h2 + (TB - RR - a1)));
-        out[i] = static_cast<uint16_t>(labs_energy_u32(static_cast<uint32_t>(base + off), n));
+    static_assert(NR % 8 == 0, "codes are packed eight at a time");
+    for (uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; g < dim / 8;
+         g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        uint32_t m[8];
+        for (int q = 0; q < 8; ++q) {
+            const uint64_t i = 8 * g + q;
+            const uint64_t tile = i >> TB;
+            const uint32_t rem = static_cast<uint32_t>(i) & ((1u << TB) - 1u);
+            const uint32_t t = rem >> RB, j = rem & (NR - 1u);
+            uint32_t u = 0;
+            for (int b = 0; b < TT; ++b) u |= ((t >> b) & 1u) << P.ph[NP - 1].thr[b];
+            for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
+            const uint32_t ua = u >> RR;
+            const uint64_t off = static_cast<uint64_t>(u & ROWMASK) +
+                                 (static_cast<uint64_t>(ua & ((1u << a1) - 1u)) << h) +
+                                 (static_cast<uint64_t>(ua >> a1) << h2);
+            const uint64_t lo = tile & ((1ull << lo_bits) - 1ull);
+            const uint64_t mid = (tile >> lo_bits) & ((1ull << mid_bits) - 1ull);
+            const uint64_t hi = tile >> (lo_bits + mid_bits);
+            const uint64_t base = (lo << RR) | (mid << (h + a1)) | (hi << (h2 + (TB - RR - a1)));
+            m[q] = labs_energy_u32(static_cast<uint32_t>(base + off), n) >> 2;
+        }
+        pack8(out + 3 * g, m);
     }
 }
 
@@ -2086,10 +2142,11 @@ void fill_high_r(int rbits, G& go)
     }
 }
 
-hipError_t launch_energy_fill_high(uint16_t* out, int n, int precision, const HighSweep& hs, hipStream_t stream)
+hipError_t launch_energy_fill_high(uint32_t* out, int n, int precision, const HighSweep& hs, hipStream_t stream)
 {
     const uint64_t dim = 1ull << n;
-    const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 65536));
+    if (n < 3) return hipErrorInvalidValue;  // codes go in groups of eight
+    const int grid = static_cast<int>(std::min<uint64_t>((dim / 8 + kThreads - 1) / kThreads, 65536));
     const int lo = hs.h1 - hs.R, mid = hs.h2 - (hs.h1 + hs.a1);
     hipError_t e = hipErrorInvalidValue;
     auto go = [&](auto kernel) {
@@ -2247,10 +2304,11 @@ hipError_t launch_carry_reset(double* carry, int count, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream, int low_bits)
+hipError_t launch_energy_fill(uint32_t* out, int n, int precision, hipStream_t stream, int low_bits)
 {
     const uint64_t dim = 1ull << n;
-    const int grid = static_cast<int>(std::min<uint64_t>((dim + kThreads - 1) / kThreads, 65536));
+    if (n < 3) return hipErrorInvalidValue;  // codes go in groups of eight
+    const int grid = static_cast<int>(std::min<uint64_t>((dim / 8 + kThreads - 1) / kThreads, 65536));
     auto k64 = k_energy_fill<LowPlan<Cfg<float2>::TB, Cfg<float2>::RB>>;
     auto k64w = k_energy_fill<LowPlan<kWideLowTB, Cfg<float2>::RB>>;
     auto k128 = k_energy_fill<LowPlan<Cfg<double2>::TB, Cfg<double2>::RB>>;

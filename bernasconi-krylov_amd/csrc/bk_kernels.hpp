@@ -30,9 +30,10 @@ struct StepSpec {
     double* series_slot;    // device pointer to 3 doubles {P_<, <d_H>, norm} or null
     uint32_t s0;
     uint32_t e0;
-    // optional uint16 energy cache (E(s) per amplitude); null -> energies are
-    // computed on the fly with popcounts in the sweep
-    const uint16_t* ecache = nullptr;
+    // optional energy cache (12-bit codes E(s) >> 2, packed eight to three
+    // words: launch_energy_fill); null -> energies are computed on the fly
+    // with popcounts in the sweep
+    const uint32_t* ecache = nullptr;
     // Optional error-feedback phase table (fb_carry non-null): before the step's
     // phase multiply, entry m of phase_tab (writable, fb_count entries) is set
     // to round(sd * fb_master[m] * fb_carry[m]) and fb_carry[m] to the exact
@@ -103,8 +104,8 @@ struct RunSpec {
     double* fb_carry;
     int fb_count;
     uint32_t s0, e0;
-    const uint16_t* ecache;       // LOW-layout E(s) cache, filled for low_bits
-    const uint16_t* ecache_high;  // E(s) in the phase-carrying high group's layout (launch_energy_fill_high);
+    const uint32_t* ecache;       // LOW-layout E(s) cache, filled for low_bits
+    const uint32_t* ecache_high;  // E(s) in the phase-carrying high group's layout (launch_energy_fill_high);
                                   // null: step by step
     int low_bits = 0;             // LOW tile of the fused run (fused_low_bits; 0: the default tile)
 };
@@ -116,7 +117,7 @@ hipError_t launch_run(void* psi, const RunSpec& rs, const SweepTuning& tun, doub
 // fused_low_bits gives the forward run's choice)
 std::vector<HighSweep> fused_sweeps(int n, int precision, const SweepTuning& tun, int low_bits = 0);
 int fused_low_bits(int n, int precision, const SweepTuning& tun);
-hipError_t launch_energy_fill_high(uint16_t* out, int n, int precision, const HighSweep& hs, hipStream_t stream);
+hipError_t launch_energy_fill_high(uint32_t* out, int n, int precision, const HighSweep& hs, hipStream_t stream);
 
 // ---- reverse pass of the adjoint gradient ----
 // One reverse step of phi (reversed forward state) and chi (adjoint state),
@@ -140,8 +141,8 @@ struct RevRunSpec {
     const double* gdt;           // host: per reverse step
     double* const* grad_slots;   // host: device slot per reverse step
     const void* phase_tab;       // exp(-i dtp E/n) per E >> 2
-    const uint16_t* ecache;      // LOW-layout E(s)
-    const uint16_t* ecache_high; // E(s) in the phase-carrying high group's layout
+    const uint32_t* ecache;      // LOW-layout E(s)
+    const uint32_t* ecache_high; // E(s) in the phase-carrying high group's layout
     // complex64: an error-feedback table for dtp (as StepSpec's); when given
     // and every step allows it, the runs use the tan butterfly (the table
     // carries each step's c^n) instead of (c, s) with phase_tab
@@ -177,7 +178,7 @@ struct BatchGroup {
     void* psi_out;            // launch_batch_pass: output states in the same layout (null: in place)
     const void* tab;          // error-feedback tables, tab_stride entries per state
     int tab_stride;
-    const uint16_t* ecache;   // E(s) cache of n (shared by the states)
+    const uint32_t* ecache;   // E(s) cache of n (shared by the states)
     const double* bc;         // per state (device)
     const double* bs;         // per state (device)
     const uint32_t* s0;       // per state (device)
@@ -196,7 +197,7 @@ hipError_t launch_batch_step(const BatchGroup& g, const SweepTuning& tun, double
 struct BatchPass {
     enum Kind { kLowPlain, kHighPlain, kLowPhase, kHighPhase } kind;
     HighSweep hs;                  // high passes
-    const uint16_t* ecache_high;   // kHighPhase: E(s) in hs's layout (launch_energy_fill_high)
+    const uint32_t* ecache_high;   // kHighPhase: E(s) in hs's layout (launch_energy_fill_high)
     bool measure, post;
 };
 hipError_t launch_batch_pass(const BatchGroup& g, const BatchPass& bp, const SweepTuning& tun, double* partials,
@@ -226,10 +227,14 @@ hipError_t launch_init_basis(void* psi, int n, int precision, uint64_t s0, doubl
 // fb_carry[m] = 1 for m < count (start of a new error-feedback history)
 hipError_t launch_carry_reset(double* carry, int count, hipStream_t stream);
 
-// E(s) for all 2^n states (uint16, exact for n <= 32) in the LOW sweep's tile
-// layout for 'precision' (the layout differs between c64 and c128)
+// Bytes of an energy cache of n qubits: a 12-bit code m = E(s) >> 2 per state
+// (E(s) = 4m + (n/2 mod 4) exactly; m <= 2604 for n <= 32), eight codes in
+// three 32-bit words
+inline size_t energy_cache_bytes(int n) { return (static_cast<size_t>(3) << n) / 2; }
+// The energy codes of all 2^n states (n >= 3) in the LOW sweep's tile layout
+// for 'precision' (the layout differs between c64 and c128)
 // low_bits: the LOW tile whose layout the cache follows (0: the default tile)
-hipError_t launch_energy_fill(uint16_t* out, int n, int precision, hipStream_t stream, int low_bits = 0);
+hipError_t launch_energy_fill(uint32_t* out, int n, int precision, hipStream_t stream, int low_bits = 0);
 
 // count of s with E(s) < e0 over all 2^n states (exact, integer)
 hipError_t launch_count_below(int n, uint32_t e0, unsigned long long* d_count,

@@ -410,8 +410,9 @@ void bk_result_free(bk_sampling_result* r);
 
 /* ---- how the first sweep of a step obtains E(s) (the reference caches E as FP64,
  * 8 B per amplitude, sampler.cuh:54-70, 320-338) ----
- * BK_ENERGY_CACHE16 (default): a uint16 E(s) array (2 B/amplitude, exact) filled
- *   once per n and read by the sweep;
+ * BK_ENERGY_CACHE16 (default): an exact E(s) cache filled once per n and read by
+ *   the sweep, 1.5 B/amplitude (12-bit codes E >> 2; E mod 4 is fixed by n; the
+ *   name is kept from its first, 16-bit layout);
  * BK_ENERGY_ONTHEFLY: recomputed every step from popcounts (no extra memory). */
 #define BK_ENERGY_ONTHEFLY 0
 #define BK_ENERGY_CACHE16 1
