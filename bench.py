@@ -235,7 +235,8 @@ def _batch_bench(bk, eng, seed: int) -> dict:
     serial baseline runs 8 instances of each n one bk_evolve at a time (same
     kernels, observables at the end) and scales by the instance counts; each
     n's instances also run as a batch of their own (per-n ratio and HBM rate,
-    counting 18 B per amplitude and step: state read and written, uint16 E)."""
+    counting 17.5 B per amplitude and step: state read and written, the 12-bit
+    energy codes)."""
     import numpy as np
     steps, dt = 500, 0.02
     inst = bk.batch_workload(512, seed=seed)
@@ -262,7 +263,7 @@ def _batch_bench(bk, eng, seed: int) -> dict:
         rs, sst = eng.batch_run(sub, dt, steps, use_graph=True, autotune=True)
         tb = time.perf_counter() - t0
         identical &= bool(np.array_equal(rs, res[idx]))
-        step_bytes = len(idx) * float(1 << n) * 18.0  # state read + written (c64) + uint16 energy
+        step_bytes = len(idx) * float(1 << n) * 17.5  # state read + written (c64) + 1.5 B energy code
         sample[str(n)] = {"instances": len(idx), "serial_s_per_instance": per,
                           "serial_s_estimated": per * len(idx), "batched_s": tb,
                           "batched_over_serial": per * len(idx) / tb,
