@@ -138,25 +138,54 @@ def test_c64_against_c128_n24(engine, bk):
     assert 1.0 - abs(np.vdot(psi128, psi64)) <= 1e-5
 
 
-@pytest.mark.parametrize("n,prec", [(14, C128), (20, C64), (22, C128)])
-def test_energy_modes_bit_identical(engine, orc, bk, n, prec):
-    """uint16 energy cache and on-the-fly popcount energies feed the same integer
-    E(s) to the same kernels: the series must agree bit for bit.  On-the-fly
-    energies run the three-pass step schedule, so the cached run takes it too."""
+# n mod 8 covers every residue c_n = (n / 2) mod 4 of E(s) mod 4
+@pytest.mark.parametrize("n,prec", [(14, C128), (16, C64), (18, C128), (20, C64), (22, C128)])
+@pytest.mark.parametrize("shift", [0, 1, 2])
+def test_energy_modes_bit_identical(engine, orc, bk, n, prec, shift):
+    """The energy-code cache (12-bit E >> 2) and on-the-fly popcount energies
+    feed the same phases and the same P_< test to the same kernels: the series
+    must agree bit for bit.  On-the-fly energies run the three-pass step
+    schedule, so the cached run takes it too.  shift moves the P_< threshold
+    off the lattice E = 4m + c_n that every energy lies on (e0 + shift), where
+    the cached test m < ceil((e0 - c_n) / 4) must still equal E < e0."""
     _, s0, e0 = orc.make_trap(n, 31)
+    e0 += shift
     engine.set_tuning("fused", 0)
     try:
         engine.set_energy_mode(bk.BK_ENERGY_ONTHEFLY)
-        a = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
+        a = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec, e0=e0)
         psi_a = engine.download()
         engine.set_energy_mode(bk.BK_ENERGY_CACHE16)
-        b = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec)
+        b = engine.evolve(n, s0, 0.35, 0.02, 15, order=STRANG, precision=prec, e0=e0)
         psi_b = engine.download()
     finally:
         engine.set_energy_mode(bk.BK_ENERGY_CACHE16)
         engine.set_tuning("fused", 1)
     assert np.array_equal(a["p_less"], b["p_less"]) and np.array_equal(a["dh"], b["dh"])
     assert np.array_equal(psi_a, psi_b)
+    assert a["p_less"][-1] > 0.0 or shift == 0
+
+
+@pytest.mark.parametrize("n", [24, 26, 28])
+@pytest.mark.parametrize("shift", [1, 3])
+def test_fused_threshold_off_lattice(engine, orc, bk, n, shift):
+    """The fused schedule reads both energy-code caches (the LOW tile's and the
+    phase-carrying high group's); with a P_< threshold off the energy lattice
+    its series matches the step-by-step schedule on on-the-fly energies in
+    complex128 (two schedules: to 1e-12, as for on-lattice thresholds)."""
+    _, s0, e0 = orc.make_trap(n, 5)
+    e0 += shift
+    a = engine.evolve(n, s0, 0.35, 0.02, 12, order=STRANG, precision=C128, e0=e0)
+    engine.set_tuning("fused", 0)
+    try:
+        engine.set_energy_mode(bk.BK_ENERGY_ONTHEFLY)
+        b = engine.evolve(n, s0, 0.35, 0.02, 12, order=STRANG, precision=C128, e0=e0)
+    finally:
+        engine.set_energy_mode(bk.BK_ENERGY_CACHE16)
+        engine.set_tuning("fused", 1)
+    assert np.max(np.abs(a["p_less"] - b["p_less"])) <= 1e-12
+    assert np.max(np.abs(a["dh"] - b["dh"])) <= 1e-11
+    assert a["p_less"][-1] > 0.0
 
 
 @pytest.mark.parametrize("fused", [0, 1])
