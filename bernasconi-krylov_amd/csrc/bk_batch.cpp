@@ -350,7 +350,12 @@ static int bk_batch_run_impl(bk_ctx* c, const bk_instance* inst, int count, cons
         P.ecache[g.n] = ec.as<uint32_t>();
         // fused schedule (default tuning: the tuner's row bits steer step-by-step sweeps only)
         std::vector<bk::HighSweep> fg;
-        if (g.tan && c->tuning.fused) fg = bk::fused_sweeps(g.n, prec, bk::default_tuning(g.n, prec));
+        if (g.tan && c->tuning.fused) {
+            // batched passes run every high group on 2^HTB-amplitude tiles
+            bk::SweepTuning bt = bk::default_tuning(g.n, prec);
+            bt.narrow_phase = false;
+            fg = bk::fused_sweeps(g.n, prec, bt);
+        }
         g.fused = fg;
         if (!fg.empty()) {
             const bk::HighSweep& h = fg.back();

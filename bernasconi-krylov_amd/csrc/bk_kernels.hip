@@ -75,6 +75,27 @@ __device__ __forceinline__ void store_amp(V* p, const V& v)
     }
 }
 
+// Amplitude load of a sweep.  NT: a nontemporal load (streamed through the
+// caches without keeping the line), which leaves the vector L1 to the
+// phase-table gathers of one-state passes that read the table from global
+// memory (N=30 LOW phase pass 3.645 -> 3.612 ms; two-state passes are slower
+// with it: 0.560 -> 0.575 ms at N=26).
+template <bool NT, typename V>
+__device__ __forceinline__ V load_amp(const V* p)
+{
+    if constexpr (NT) {
+        using R = typename RealOf<V>::T;
+        typedef R N __attribute__((ext_vector_type(2)));
+        const N w = __builtin_nontemporal_load(reinterpret_cast<const N*>(p));
+        V v;
+        v.x = w.x;
+        v.y = w.y;
+        return v;
+    } else {
+        return *p;
+    }
+}
+
 template <typename V, typename R>
 __device__ __forceinline__ void bfly_tan(V& a, V& b, R t)
 {
@@ -223,13 +244,13 @@ __host__ __device__ constexpr int phase_tab_count(int n)
 // the exchange buffer (at most 2605 entries, 20.8 KB at n = 32): the
 // per-amplitude table reads become LDS gathers instead of scattered global
 // loads through the vector memory pipeline the state streams use.  The
-// 2^13-amplitude LOW tile keeps the table in global memory, where the extra
-// LDS would halve its workgroups per CU.  Batched passes keep per-state
-// tables in global memory.
+// 2^13-amplitude tiles (LOW, and the narrow phase-carrying high group) keep
+// the table in global memory, where the extra LDS would halve their
+// workgroups per CU.  Batched passes keep per-state tables in global memory.
 template <typename V, bool LOW, int TB, int FLAGS>
 constexpr bool lds_phase_table()
 {
-    return (!LOW || TB >= 14) && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 && sizeof(V) == 8;
+    return TB >= 14 && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 && sizeof(V) == 8;
 }
 
 template <int TB_, int RB_, int R_>
@@ -261,6 +282,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     static_assert(!BATCH || (NS == 1 && (FLAGS & F_PREFETCH) == 0), "batched sweeps: one state per tile, no prefetch");
     constexpr bool LOW = P.low;
     constexpr bool NTSTORE = !BATCH && (LOW || (FLAGS & F_PHASE) == 0);
+    constexpr bool NTLOAD = !BATCH && NS == 1 && (FLAGS & F_PHASE) != 0 && !lds_phase_table<V, P.low, TB, FLAGS>();
     V* sp[NS];
     V* dp[NS];
     sp[0] = psi;
@@ -376,7 +398,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             for (int s = 0; s < NS; ++s) {
                 const V* tp = sp[s] + base + tg0;
 #pragma unroll
-                for (int j = 0; j < NR; ++j) xv[s][j] = tp[reg_off(str, j)];
+                for (int j = 0; j < NR; ++j) xv[s][j] = load_amp<NTLOAD>(tp + reg_off(str, j));
             }
         } else {
 #pragma unroll
@@ -385,7 +407,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
                     const uint32_t pj = reg_pattern(P.ph[0], RB, j);
-                    xv[s][j] = tp[tg0 + gl(pj)];
+                    xv[s][j] = load_amp<NTLOAD>(tp + tg0 + gl(pj));
                 }
             }
         }
@@ -1512,6 +1534,12 @@ hipError_t batch_step_impl(const BatchGroup& g, const SweepTuning& tun, double* 
 // row bits of a phase-carrying high pass: 3..12 covers every n whose high
 // qubits fit one or two sweep groups (n <= 32)
 constexpr int kFusedRMin = 3, kFusedRMax = 12;
+// the narrow phase-carrying high group (SweepTuning::narrow_phase): complex64
+// tiles of 2^kNarrowTB amplitudes with rows of 2^6 .. 2^8 amplitudes
+// (n >= kNarrowMinN: states beyond the last-level cache; the batched passes,
+// whose states are smaller, keep 2^14-amplitude tiles, so a batched state of
+// n < kNarrowMinN follows its serial run bit for bit, observables included)
+constexpr int kNarrowTB = 13, kNarrowRMin = 6, kNarrowRMax = 8, kNarrowMinN = 26;
 
 // LTB: tile bits of the LOW pass (the default tile, or the complex64 wide tile
 // of 2^kWideLowTB amplitudes, 512 threads, that a fused run can take)
@@ -1536,11 +1564,11 @@ hipError_t launch_low_run(int ltb, V* psi, const SweepArgs& a, int grid, hipStre
     return launch_low_fl<V, FL>(psi, a, grid, st);
 }
 
-template <typename V, int R, int FL>
+template <typename V, int R, int FL, int HTB = Cfg<V>::HTB>
 hipError_t launch_high_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    using PH = HighPlan<C::HTB, C::HRB, R>;
+    using PH = HighPlan<HTB, C::HRB, R>;
     auto k = k_sweep<V, PH, FL, C::SW>;
     hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
@@ -1557,23 +1585,46 @@ hipError_t launch_highp_r(int rbits, V* psi, const SweepArgs& a, int grid, hipSt
     }
 }
 
+// the narrow phase-carrying group: 2^kNarrowTB-amplitude tiles (complex64)
+template <typename V, int FL, int R = kNarrowRMin>
+hipError_t launch_highn_r(int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if constexpr (R > kNarrowRMax || !std::is_same_v<V, float2>) {
+        return hipErrorInvalidValue;
+    } else {
+        if (rbits == R) return launch_high_fl<V, R, FL, kNarrowTB>(psi, a, grid, st);
+        return launch_highn_r<V, FL, R + 1>(rbits, psi, a, grid, st);
+    }
+}
+
+// a phase-carrying high pass of a group on htb tile bits
+template <typename V, int FL>
+hipError_t launch_highp(int htb, int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if (htb == Cfg<V>::HTB) return launch_highp_r<V, FL>(rbits, psi, a, grid, st);
+    if (htb == kNarrowTB) return launch_highn_r<V, FL>(rbits, psi, a, grid, st);
+    return hipErrorInvalidValue;
+}
+
 // phase-carrying passes (tan butterflies): F_PHASE | F_ECACHE | F_TAN | any of
 // F_MEASURE, F_POSTROT
 template <typename V>
-hipError_t launch_ppass(bool low, int ltb, int fl, int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
+hipError_t launch_ppass(bool low, int ltb, int fl, int htb, int rbits, V* psi, const SweepArgs& a, int grid,
+                        hipStream_t st)
 {
     constexpr int B = F_PHASE | F_ECACHE | F_TAN;
     switch (fl) {
-    case B: return low ? launch_low_run<V, B>(ltb, psi, a, grid, st) : launch_highp_r<V, B>(rbits, psi, a, grid, st);
+    case B:
+        return low ? launch_low_run<V, B>(ltb, psi, a, grid, st) : launch_highp<V, B>(htb, rbits, psi, a, grid, st);
     case B | F_MEASURE:
         return low ? launch_low_run<V, B | F_MEASURE>(ltb, psi, a, grid, st)
-                   : launch_highp_r<V, B | F_MEASURE>(rbits, psi, a, grid, st);
+                   : launch_highp<V, B | F_MEASURE>(htb, rbits, psi, a, grid, st);
     case B | F_POSTROT:
         return low ? launch_low_run<V, B | F_POSTROT>(ltb, psi, a, grid, st)
-                   : launch_highp_r<V, B | F_POSTROT>(rbits, psi, a, grid, st);
+                   : launch_highp<V, B | F_POSTROT>(htb, rbits, psi, a, grid, st);
     case B | F_MEASURE | F_POSTROT:
         return low ? launch_low_run<V, B | F_MEASURE | F_POSTROT>(ltb, psi, a, grid, st)
-                   : launch_highp_r<V, B | F_MEASURE | F_POSTROT>(rbits, psi, a, grid, st);
+                   : launch_highp<V, B | F_MEASURE | F_POSTROT>(htb, rbits, psi, a, grid, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1583,7 +1634,7 @@ hipError_t launch_ppass(bool low, int ltb, int fl, int rbits, V* psi, const Swee
 // with rows of at least 2^kFusedRMin amplitudes, the last group (the
 // phase-carrying one) with R in [kFusedRMin, kFusedRMax].
 template <typename V>
-std::vector<HighSweep> fused_groups(int n, const SweepTuning& tun, int ltb = 0)
+std::vector<HighSweep> planned_groups(int n, const SweepTuning& tun, int ltb)
 {
     using C = Cfg<V>;
     const int tb = ltb > 0 ? ltb : C::TB;
@@ -1601,6 +1652,27 @@ std::vector<HighSweep> fused_groups(int n, const SweepTuning& tun, int ltb = 0)
     return {};
 }
 
+// tile bits of a high group's passes: R row bits and its rotated qubits
+inline int group_tb(const HighSweep& hs) { return hs.R + hs.a1 + hs.a2; }
+
+// planned_groups, with the phase-carrying group of a two-group complex64 run
+// moved to 2^kNarrowTB-amplitude tiles when SweepTuning::narrow_phase asks
+// for it, n >= kNarrowMinN and its rows stay at least 2^kNarrowRMin
+// amplitudes long; or to 1024-thread passes when SweepTuning::phase_rb asks
+template <typename V>
+std::vector<HighSweep> fused_groups(int n, const SweepTuning& tun, int ltb = 0)
+{
+    std::vector<HighSweep> g = planned_groups<V>(n, tun, ltb);
+    if constexpr (std::is_same_v<V, float2>) {
+        if (tun.narrow_phase && g.size() == 2 && n >= kNarrowMinN) {
+            HighSweep& b = g.back();
+            const int r = b.R - (group_tb(b) - kNarrowTB);
+            if (group_tb(b) > kNarrowTB && r >= kNarrowRMin && r <= kNarrowRMax) b.R = r;
+        }
+    }
+    return g;
+}
+
 // LOW tile bits of a fused forward run.  complex64 at n = 32 has 19 high
 // qubits: beside a 13-qubit LOW tile two groups of at most 14 - R qubits need
 // 16-amplitude rows (R = 4, 128 B) in one of them, and short rows cost HBM
@@ -1613,11 +1685,11 @@ int fused_ltb(int n, const SweepTuning& tun)
     constexpr int kFullRows = 5;  // 32 amplitudes, 256 B
     if constexpr (std::is_same_v<V, float2>) {
         if (tun.wide_low) {
-            const std::vector<HighSweep> narrow = fused_groups<V>(n, tun);
+            const std::vector<HighSweep> narrow = planned_groups<V>(n, tun, 0);
             bool short_rows = false;
             for (const HighSweep& h : narrow) short_rows |= h.R < kFullRows;
             if (short_rows && narrow.size() == 2) {
-                const std::vector<HighSweep> wide = fused_groups<V>(n, tun, kWideLowTB);
+                const std::vector<HighSweep> wide = planned_groups<V>(n, tun, kWideLowTB);
                 bool ok = wide.size() == 2;
                 for (const HighSweep& h : wide) ok &= h.R >= kFullRows;
                 if (ok) return kWideLowTB;
@@ -1744,7 +1816,7 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0);
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
-        e = launch_ppass<V>(low, ltb, fl, B.R, psi, b, g, st);
+        e = launch_ppass<V>(low, ltb, fl, group_tb(B), B.R, psi, b, g, st);
         if (e != hipSuccess) return e;
         if (tun.timer) tun.timer->end(st, low ? "fused_low_phase" : "fused_high_phase", pass_bytes);
         if (slot) {
@@ -1758,19 +1830,31 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
 // ---- fused reverse runs (adjoint): the schedule of run_impl on two states ----
 constexpr int kRevPass = F_PHASE | F_GRAD | F_ECACHE | F_POSTROT;
 
-template <typename V, int FL = kRevPass, int R = kFusedRMin>
-hipError_t launch_high_rev_ppass(V* phi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+template <typename V, int FL = kRevPass, int R = kFusedRMin, int HTB = Cfg<V>::HTB>
+hipError_t launch_high_rev_ppass_r(V* phi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
-    if constexpr (R > kFusedRMax || R >= C::HTB) {
+    constexpr int RMAX = HTB == Cfg<V>::HTB ? kFusedRMax : kNarrowRMax;
+    if constexpr (R > RMAX || R >= HTB) {
         return hipErrorInvalidValue;
     } else {
-        if (rbits != R) return launch_high_rev_ppass<V, FL, R + 1>(phi, rbits, a, grid, st);
-        using PH = HighPlan<C::HTB, C::HRB, R>;
+        if (rbits != R) return launch_high_rev_ppass_r<V, FL, R + 1, HTB>(phi, rbits, a, grid, st);
+        using PH = HighPlan<HTB, C::HRB, R>;
         auto k = k_sweep<V, PH, FL, C::SW, 2>;
         hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, phi, a);
         return hipGetLastError();
     }
+}
+
+// reverse phase pass of a high group on htb tile bits
+template <typename V, int FL = kRevPass>
+hipError_t launch_high_rev_ppass(V* phi, int htb, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+{
+    if (htb == Cfg<V>::HTB) return launch_high_rev_ppass_r<V, FL>(phi, rbits, a, grid, st);
+    if constexpr (std::is_same_v<V, float2>) {
+        if (htb == kNarrowTB) return launch_high_rev_ppass_r<V, FL, kNarrowRMin, kNarrowTB>(phi, rbits, a, grid, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 // Reverse steps i = 0..K-1 of phi and chi, each the rotations exp(i gdt_i X)
@@ -1859,12 +1943,16 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         }
         return phase ? launch_low_rev<V, kRevPass>(phi, b, gL, st) : launch_low_rev<V, F_GRAD>(phi, b, gL, st);
     };
-    auto high_rev = [&](bool phase, int R, const SweepArgs& b) -> hipError_t {
+    // phase passes run on the group's own tile (the narrow phase-carrying
+    // group included); plain passes on 2^HTB-amplitude tiles
+    auto high_rev = [&](bool phase, const HighSweep& hs, const SweepArgs& b) -> hipError_t {
+        const int R = hs.R, htb = group_tb(hs);
+        if (!phase && htb != C::HTB) return hipErrorInvalidValue;
         if constexpr (std::is_same_v<V, float2>) {
-            if (tan) return phase ? launch_high_rev_ppass<V, kRevPass | F_TAN>(phi, R, b, gH, st)
+            if (tan) return phase ? launch_high_rev_ppass<V, kRevPass | F_TAN>(phi, htb, R, b, gH, st)
                                   : launch_high_rev<V, F_GRAD | F_TAN>(phi, R, b, gH, st);
         }
-        return phase ? launch_high_rev_ppass<V>(phi, R, b, gH, st) : launch_high_rev<V>(phi, R, b, gH, st);
+        return phase ? launch_high_rev_ppass<V>(phi, htb, R, b, gH, st) : launch_high_rev<V>(phi, R, b, gH, st);
     };
     if (two) {
         SweepArgs b = a;
@@ -1878,7 +1966,7 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         coef(0, &b.bc, &b.bs);
         weights(b, 0, 0, nB);
         b.partials = region(0) + offB;
-        e = timed("adjoint_open", [&] { return high_rev(false, B.R, b); });
+        e = timed("adjoint_open", [&] { return high_rev(false, B, b); });
     }
     if (e != hipSuccess) return e;
     for (int i = 0; i < K; ++i) {
@@ -1888,7 +1976,7 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
             coef(i, &b.bc, &b.bs);
             weights(b, i, even ? nL : nB, nA);
             b.partials = region(i) + offA;
-            if ((e = timed("adjoint_high_a", [&] { return high_rev(false, A.R, b); })) != hipSuccess) return e;
+            if ((e = timed("adjoint_high_a", [&] { return high_rev(false, A, b); })) != hipSuccess) return e;
         }
         if (tan) {
             // step i's table: P(-dt) times the exact c^n of its rotations, with feedback
@@ -1923,7 +2011,7 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         b.partials = region(i) + off;
         b.partials2 = region(i + 1) + off;
         e = low ? timed("adjoint_low_phase", [&] { return low_rev(true, b); })
-                : timed("adjoint_high_phase", [&] { return high_rev(true, B.R, b); });
+                : timed("adjoint_high_phase", [&] { return high_rev(true, B, b); });
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, region(i), GS, rs.grad_slots[i]);
         if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -2023,6 +2111,10 @@ SweepTuning default_tuning(int n, int precision)
     t.grid = 4096;
     t.pingpong = false;
     t.wide_low = true;
+    // narrow phase group: N=26 complex64 B phase pass 0.283 -> 0.237 ms (step
+    // 0.460 -> 0.436 ms), N=27 0.485 -> 0.465, N=28 0.918 -> 0.899; the
+    // adjoint's reverse B phase pass at N=26 0.513 -> 0.460 ms
+    t.narrow_phase = true;
     t.prefetch_low = false;
     t.prefetch_high = false;
     // page-balancing granule of the high groups' planner, in amplitudes: 2^20
@@ -2133,12 +2225,13 @@ hipError_t launch_run(void* psi, const RunSpec& rs, const SweepTuning& tun, doub
     return hipErrorInvalidValue;
 }
 
-template <class C, int R = kFusedRMin, typename G>
+template <class C, int HTB = C::HTB, int R = kFusedRMin, typename G>
 void fill_high_r(int rbits, G& go)
 {
-    if constexpr (R <= kFusedRMax && R < C::HTB) {
-        if (rbits == R) go(k_energy_fill_high<HighPlan<C::HTB, C::HRB, R>>);
-        else fill_high_r<C, R + 1>(rbits, go);
+    constexpr int RMAX = HTB == C::HTB ? kFusedRMax : kNarrowRMax;
+    if constexpr (R <= RMAX && R < HTB) {
+        if (rbits == R) go(k_energy_fill_high<HighPlan<HTB, C::HRB, R>>);
+        else fill_high_r<C, HTB, R + 1>(rbits, go);
     }
 }
 
@@ -2153,8 +2246,13 @@ hipError_t launch_energy_fill_high(uint32_t* out, int n, int precision, const Hi
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, out, n, hs.h1, hs.a1, hs.h2, lo, mid, dim);
         e = hipGetLastError();
     };
-    if (precision == kC64) fill_high_r<Cfg<float2>>(hs.R, go);
-    else fill_high_r<Cfg<double2>>(hs.R, go);
+    const int htb = group_tb(hs);
+    if (precision == kC64) {
+        if (htb == Cfg<float2>::HTB) fill_high_r<Cfg<float2>>(hs.R, go);
+        else if (htb == kNarrowTB) fill_high_r<Cfg<float2>, kNarrowTB, kNarrowRMin>(hs.R, go);
+    } else if (htb == Cfg<double2>::HTB) {
+        fill_high_r<Cfg<double2>>(hs.R, go);
+    }
     return e;
 }
 

@@ -66,6 +66,11 @@ struct SweepTuning {
     int page_bits;       // > 0: deal qubits >= page_bits evenly over the HIGH sweeps
     bool fused = true;   // runs of steps use the two-pass schedule when the plan allows
     bool wide_low = true;  // complex64 fused runs may take a 14-qubit LOW tile (n = 32: full rows in both high groups)
+    // complex64 fused runs with two high groups: the phase-carrying group B
+    // moves on 2^13-amplitude tiles (one row bit fewer, 256 threads, 64 KB of
+    // LDS, phase table in global memory), two workgroups per CU, when its rows
+    // keep at least 64 amplitudes (default_tuning: on)
+    bool narrow_phase = false;
     KernelTimer* timer = nullptr;
 };
 

@@ -435,6 +435,12 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  * "wide_low" : 1 (default) = a complex64 fused run whose two high groups would
  *              need 16-amplitude rows (n = 32) takes a 14-qubit LOW tile instead,
  *              so both groups keep 32-amplitude rows; 0 = the 13-qubit tile
+ * "narrow_phase" : 1 (default) = in a complex64 fused run with two high groups,
+ *              the phase-carrying group moves on 2^13-amplitude tiles (two
+ *              workgroups per CU) when its rows stay at least 64 amplitudes
+ *              long (n = 26..28; smaller states stay in the last-level cache);
+ *              0 = 2^14-amplitude tiles.  The state is the
+ *              same bit for bit either way.
  * A key keeps its value for the context until set again; every other key takes
  * the defaults of each new state's n and precision. */
 int bk_set_tuning(bk_ctx* ctx, const char* key, int value);

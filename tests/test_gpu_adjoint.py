@@ -215,6 +215,33 @@ def test_fused_reverse_matches_step_by_step(bk, n, prec):
     assert d1 < (1e-20 if prec == "c128" else 1e-9)
 
 
+def test_narrow_phase_group_reverse(bk):
+    """n = 26 complex64 (the differentiation workload's size): the adjoint with
+    the phase-carrying group on 2^13-amplitude tiles (the default) against
+    2^14-amplitude tiles.  The forward observables agree to FP64 summation
+    order, the states bit for bit (so the drift is the same), and the
+    gradients to 1e-6 of the largest component (per-thread FP32 partials of
+    the pair terms group differently on the two tilings)."""
+    n = 26
+    _, s0, e0 = bk.make_trap(n, 7)
+    gs = np.repeat(_fields(4, 91), 10)
+    out = {}
+    for narrow in (0, 1):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("narrow_phase", narrow)
+            obs = eng.adjoint_forward(n, s0, gs, 0.02, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                                      checkpoint_every=16)
+            grad, drift = eng.adjoint_backward(1.0, 0.25)
+        finally:
+            eng.close()
+        out[narrow] = (np.asarray(obs), grad, drift)
+    (o0, g0, d0), (o1, g1, d1) = out[0], out[1]
+    assert np.abs(o1 - o0).max() <= 1e-10
+    assert d0 == d1
+    assert np.abs(g1 - g0).max() <= 1e-6 * np.abs(g0).max()
+
+
 def test_backward_leaves_state_and_detects_stale_tape(bk, engine):
     n = 13
     _, s0, e0 = bk.make_trap(n, 1)

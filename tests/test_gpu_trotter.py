@@ -246,6 +246,30 @@ def test_fused_run_matches_step_by_step(bk, n):
     assert np.abs(c["dh"] - a["dh"]).max() <= 1e-5
 
 
+@pytest.mark.parametrize("n", [26, 27, 28])
+def test_narrow_phase_group_bit_identical(bk, n):
+    """The phase-carrying high group of a complex64 run on 2^13-amplitude tiles
+    (the default at n = 26..28) against 2^14-amplitude tiles: every butterfly,
+    phase product and store is the same operation on the same operands, so
+    the state after the run is equal bit for bit; the recorded observables
+    differ only by the order of their FP64 partial sums."""
+    _, s0, e0 = bk.make_trap(n, 3)
+    out = {}
+    for narrow in (0, 1):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("narrow_phase", narrow)
+            r = eng.evolve(n, s0, 0.35, 0.02, 12, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                           measure_every=1)
+            out[narrow] = (r, eng.download())
+        finally:
+            eng.close()
+    (ra, pa), (rb, pb) = out[0], out[1]
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    for key, tol in (("p_less", 1e-12), ("dh", 1e-10), ("norm", 1e-12)):
+        assert np.abs(ra[key] - rb[key]).max() <= tol, key
+
+
 def test_tuning_key_keeps_other_defaults(bk):
     """bk_set_tuning pins only the key it sets: a later state of another n or
     precision takes that state's defaults for every other key (row bits,
