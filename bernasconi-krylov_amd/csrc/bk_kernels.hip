@@ -442,20 +442,43 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
 
     V x[NS][NR];
     uint32_t ec[NEC];
-    // F_GRAD: Im conj(chi_j) phi_j' + Im conj(chi_j') phi_j over the pairs of bit b
+    // F_GRAD: Im conj(chi_j) phi_j' + Im conj(chi_j') phi_j over the pairs of bit b.
+    // Phase passes (POST): two FMA chains, (sum chi.x phi'.y) - (sum chi.y phi'.x),
+    // four FMAs per pair (N=26 two-state LOW phase pass 0.550 -> 0.533 ms).
+    // Plain passes: one multiply and three FMAs per pair, independent across
+    // pairs, then a pairwise tree (the chains slow them: 0.388 -> 0.421 ms).
     auto grad_terms = [&](int b) -> R {
-        R g = R(0);
-        if constexpr (NS == 2) {
+        if constexpr (NS == 2 && POST) {
+            R gp = R(0), gm = R(0);
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
                 if ((j >> b) & 1) continue;
                 const V c0 = x[1][j], c1 = x[1][j | (1 << b)];
                 const V f0 = x[0][j], f1 = x[0][j | (1 << b)];
-                g += fma(c0.x, f1.y, -c0.y * f1.x);
-                g += fma(c1.x, f0.y, -c1.y * f0.x);
+                gp = fma(c0.x, f1.y, gp);
+                gp = fma(c1.x, f0.y, gp);
+                gm = fma(c0.y, f1.x, gm);
+                gm = fma(c1.y, f0.x, gm);
             }
+            return gp - gm;
+        } else if constexpr (NS == 2) {
+            R d[NR / 2];
+            int k = 0;
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                if ((j >> b) & 1) continue;
+                const V c0 = x[1][j], c1 = x[1][j | (1 << b)];
+                const V f0 = x[0][j], f1 = x[0][j | (1 << b)];
+                d[k++] = fma(-c1.y, f0.x, fma(-c0.y, f1.x, fma(c1.x, f0.y, c0.x * f1.y)));
+            }
+#pragma unroll
+            for (int w = NR / 4; w >= 1; w /= 2)
+#pragma unroll
+                for (int i = 0; i < w; ++i) d[i] += d[i + w];
+            return d[0];
+        } else {
+            return R(0);
         }
-        return g;
     };
     // Two-state sweeps load each tile at the top of its iteration.  Loading
     // the next tile at the bottom (as the one-state sweeps do) makes the loop
@@ -774,14 +797,28 @@ __global__ __launch_bounds__(kThreads) void k_elementwise(V* __restrict__ psi, u
 }
 
 // sum 'count' triples of partials in a fixed order into out[0..2]
+// The finishing sums load kFinishBatch strided partials per thread before
+// adding them in index order: the loads overlap instead of one L2 round trip
+// per partial (a 12288-partial gradient sum took 19 us as a dependent chain).
+constexpr int kFinishBatch = 8;
+
 __device__ __forceinline__ void finish3_body(const double* __restrict__ partials, int count, double* __restrict__ out,
                                              double* scratch)
 {
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < count; i += blockDim.x) {
-        acc[0] += partials[3 * i + 0];
-        acc[1] += partials[3 * i + 1];
-        acc[2] += partials[3 * i + 2];
+    const int bd = static_cast<int>(blockDim.x);
+    for (int i = threadIdx.x; i < count; i += kFinishBatch * bd) {
+        double v[kFinishBatch][3];
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k) {
+            const int ik = i + k * bd;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[k][c] = ik < count ? partials[3 * ik + c] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] += v[k][c];
     }
     block_reduce_store<3>(acc, scratch, out);
 }
@@ -983,7 +1020,14 @@ __global__ __launch_bounds__(kThreads) void k_finish_sum(const double* __restric
 {
     __shared__ double scratch[4];
     double acc[1] = {0.0};
-    for (int i = threadIdx.x; i < count; i += blockDim.x) acc[0] += partials[i];
+    const int bd = static_cast<int>(blockDim.x);
+    for (int i = threadIdx.x; i < count; i += kFinishBatch * bd) {
+        double v[kFinishBatch];
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k) v[k] = i + k * bd < count ? partials[i + k * bd] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k) acc[0] += v[k];
+    }
     block_reduce_store<1>(acc, scratch, out);
 }
 
