@@ -186,7 +186,8 @@ int get_fb_tab(bk_ctx* c, int n, int precision, double dtp, bk_ctx::FbTab** out)
         bk_ctx::FbTab t;
         HIPCALL(t.master.ensure(h.size() * sizeof(double)));
         HIPCALL(t.carry.ensure(h.size() * sizeof(double)));
-        HIPCALL(t.applied.ensure(static_cast<size_t>(cnt) * amp_bytes(precision)));
+        // two buffers: a fused run's phase pass of step k writes step k+1's table into the other
+        HIPCALL(t.applied.ensure(2 * static_cast<size_t>(cnt) * amp_bytes(precision)));
         HIPCALL(hipMemcpy(t.master.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         t.count = cnt;
         it = c->fb_tabs.emplace(key, t).first;
@@ -763,6 +764,7 @@ static int enqueue_run(bk_ctx* c, double dt, const std::vector<double>& gdt, con
     rs.gdt = gdt.data();
     rs.slots = slots.data();
     rs.phase_tab = fb->applied.p;
+    rs.phase_tab2 = static_cast<char*>(fb->applied.p) + static_cast<size_t>(fb->count) * amp_bytes(c->precision);
     rs.fb_master = fb->master.as<double>();
     rs.fb_carry = fb->carry.as<double>();
     rs.fb_count = fb->count;
@@ -968,6 +970,7 @@ static int bk_adjoint_backward_impl(bk_ctx* c, double w_pless, double w_dh, doub
         if ((rc = get_fb_tab(c, n, prec, -t.dt, &fb))) return rc;
         HIPCALL(bk::launch_carry_reset(fb->carry.as<double>(), fb->count, c->stream));
         rs.fb_applied = fb->applied.p;
+        rs.fb_applied2 = static_cast<char*>(fb->applied.p) + static_cast<size_t>(fb->count) * amp_bytes(prec);
         rs.fb_master = fb->master.as<double>();
         rs.fb_carry = fb->carry.as<double>();
         rs.fb_count = fb->count;

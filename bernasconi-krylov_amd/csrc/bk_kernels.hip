@@ -231,6 +231,14 @@ struct SweepArgs {
     const double* bs_arr;       // per state
     const uint32_t* s0_arr;     // per state
     const uint32_t* e0_arr;     // per state
+    // next step's error-feedback table (fb_next non-null): the first
+    // workgroups set fb_next[m] = round(fb_scale * fb_master[m] * fb_carry[m])
+    // and update the carry (feedback_entry), one entry per thread
+    void* fb_next;
+    const double* fb_master;
+    double* fb_carry;
+    int fb_count;
+    double fb_scale;
 };
 
 // entries of a phase table for n spins: E = 4m + c with E <= sum_{j<n} j^2
@@ -251,6 +259,27 @@ template <typename V, bool LOW, int TB, int FLAGS>
 constexpr bool lds_phase_table()
 {
     return TB >= 14 && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 && sizeof(V) == 8;
+}
+
+// Error-feedback phase table: applied = round(sd * master * carry),
+// carry <- exact target / applied (FP64 complex).  One entry.
+template <typename V>
+__device__ __forceinline__ void feedback_entry(const double2 w, double2* __restrict__ carry, V* __restrict__ applied,
+                                               double sd)
+{
+    using R = typename RealOf<V>::T;
+    const double2 c = *carry;
+    const double tr = sd * (w.x * c.x - w.y * c.y), ti = sd * (w.x * c.y + w.y * c.x);
+    V a;
+    a.x = static_cast<R>(tr);
+    a.y = static_cast<R>(ti);
+    const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
+    const double inv = 1.0 / (ar * ar + ai * ai);
+    double2 nc;
+    nc.x = (tr * ar + ti * ai) * inv;  // target / a
+    nc.y = (ti * ar - tr * ai) * inv;
+    *carry = nc;
+    *applied = a;
 }
 
 template <int TB_, int RB_, int R_>
@@ -489,6 +518,19 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     uint64_t tile = blockIdx.x;
     if constexpr (!TOPLOAD) {
         if (tile < a.ntiles) load(tile_base(tile), ec_base(tile, tile_base(tile)), x, ec);
+    }
+    // the next step's error-feedback table, one entry per thread of the first
+    // workgroups, while the first tile's loads are in flight (this pass reads
+    // the other table buffer; the pass that read this one has finished)
+    if constexpr ((FLAGS & F_PHASE) != 0 && !BATCH) {
+        if (a.fb_next != nullptr) {
+            const double2* w = reinterpret_cast<const double2*>(a.fb_master);
+            double2* cr = reinterpret_cast<double2*>(a.fb_carry);
+            V* nx = static_cast<V*>(a.fb_next);
+            for (int i = static_cast<int>(blockIdx.x * (1u << TT) + t); i < a.fb_count;
+                 i += static_cast<int>(gridDim.x) << TT)
+                feedback_entry<V>(w[i], cr + i, nx + i, a.fb_scale);
+        }
     }
     for (; tile < a.ntiles; tile += gridDim.x) {
         uint64_t lt = tile;
@@ -960,27 +1002,6 @@ __global__ __launch_bounds__(kThreads) void k_finish_u64(const unsigned long lon
     if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) *out = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
-}
-
-// Error-feedback phase table: applied = round(sd * master * carry),
-// carry <- exact target / applied (FP64 complex).  One entry.
-template <typename V>
-__device__ __forceinline__ void feedback_entry(const double2 w, double2* __restrict__ carry, V* __restrict__ applied,
-                                               double sd)
-{
-    using R = typename RealOf<V>::T;
-    const double2 c = *carry;
-    const double tr = sd * (w.x * c.x - w.y * c.y), ti = sd * (w.x * c.y + w.y * c.x);
-    V a;
-    a.x = static_cast<R>(tr);
-    a.y = static_cast<R>(ti);
-    const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
-    const double inv = 1.0 / (ar * ar + ai * ai);
-    double2 nc;
-    nc.x = (tr * ar + ti * ai) * inv;  // target / a
-    nc.y = (ti * ar - tr * ai) * inv;
-    *carry = nc;
-    *applied = a;
 }
 
 template <typename V>
@@ -1794,6 +1815,7 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         return hipSuccess;
     }
     const bool tan = true;
+    const bool fold = rs.phase_tab2 != nullptr;
     SweepArgs a{};
     a.tab = rs.phase_tab;
     a.partials = partials;
@@ -1839,13 +1861,15 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
             if (e != hipSuccess) return e;
             if (tun.timer) tun.timer->end(st, "fused_high_a", pass_bytes);
         }
-        // the step's error-feedback phase table
-        {
+        // the step's error-feedback phase table: its own kernel, or (two table
+        // buffers) written by the previous step's phase pass
+        V* tab_k = static_cast<V*>(fold && (k & 1) ? rs.phase_tab2 : const_cast<void*>(rs.phase_tab));
+        if (!fold || k == 0) {
             const MixCoef m = mix_coef(n, rs.gdt[k]);
             const int fg = std::max(1, std::min((rs.fb_count + kThreads - 1) / kThreads, 64));
             hipLaunchKernelGGL(k_phase_feedback<V>, dim3(fg), dim3(kThreads), 0, st,
                                reinterpret_cast<const double2*>(rs.fb_master), reinterpret_cast<double2*>(rs.fb_carry),
-                               static_cast<V*>(const_cast<void*>(rs.phase_tab)), rs.fb_count, exact_scale<Rt>(m, n));
+                               tab_k, rs.fb_count, exact_scale<Rt>(m, n));
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         // phase-carrying pass
@@ -1855,6 +1879,14 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         SweepArgs b = low ? a : high_args(a, B, dim);
         if (low) b.ntiles = dim >> ltb;
         b.ecache = low ? rs.ecache : rs.ecache_high;
+        b.tab = tab_k;
+        if (fold && post) {
+            b.fb_next = (k & 1) ? const_cast<void*>(rs.phase_tab) : rs.phase_tab2;
+            b.fb_master = rs.fb_master;
+            b.fb_carry = rs.fb_carry;
+            b.fb_count = rs.fb_count;
+            b.fb_scale = exact_scale<Rt>(mix_coef(n, rs.gdt[k + 1]), n);
+        }
         coef(k, &b.bc, &b.bs);
         if (post) coef(k + 1, &b.bc2, &b.bs2);
         const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0);
@@ -2022,19 +2054,30 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
             b.partials = region(i) + offA;
             if ((e = timed("adjoint_high_a", [&] { return high_rev(false, A, b); })) != hipSuccess) return e;
         }
-        if (tan) {
-            // step i's table: P(-dt) times the exact c^n of its rotations, with feedback
+        // step i's table: P(-dt) times the exact c^n of its rotations, with
+        // feedback; its own kernel, or (two table buffers) written by the
+        // previous step's phase pass
+        const bool fold = tan && rs.fb_applied2 != nullptr;
+        void* tab_i = fold && (i & 1) ? rs.fb_applied2 : rs.fb_applied;
+        if (tan && (!fold || i == 0)) {
             const int fg = std::max(1, std::min((rs.fb_count + kThreads - 1) / kThreads, 64));
             hipLaunchKernelGGL(k_phase_feedback<V>, dim3(fg), dim3(kThreads), 0, st,
                                reinterpret_cast<const double2*>(rs.fb_master), reinterpret_cast<double2*>(rs.fb_carry),
-                               static_cast<V*>(rs.fb_applied), rs.fb_count,
-                               exact_scale<float>(mix_coef(n, rs.gdt[i]), n));
+                               static_cast<V*>(tab_i), rs.fb_count, exact_scale<float>(mix_coef(n, rs.gdt[i]), n));
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         const bool low = two ? !even : even;
         SweepArgs b = low ? a : high_args(a, B, dim);
         if (low) b.ntiles = dim >> C::TB;
         b.ecache = low ? rs.ecache : rs.ecache_high;
+        if (tan) b.tab = tab_i;
+        if (fold && i + 1 < K) {
+            b.fb_next = (i & 1) ? rs.fb_applied : rs.fb_applied2;
+            b.fb_master = rs.fb_master;
+            b.fb_carry = rs.fb_carry;
+            b.fb_count = rs.fb_count;
+            b.fb_scale = exact_scale<float>(mix_coef(n, rs.gdt[i + 1]), n);
+        }
         coef(i, &b.bc, &b.bs);
         // the phase pass's group is the last of step i
         const int cnt = low ? nL : nB;

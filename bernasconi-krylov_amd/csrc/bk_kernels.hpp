@@ -105,6 +105,10 @@ struct RunSpec {
     const double* gdt;       // host: g dt of each step
     double* const* slots;    // host: device slot for each step's observables (entries may be null; array may be null)
     const void* phase_tab;   // error-feedback step table (see StepSpec)
+    // optional second table buffer (fb_count entries): the phase pass of step k
+    // then computes step k+1's table into the other buffer, steps alternating
+    // between phase_tab and phase_tab2, instead of a table kernel per step
+    void* phase_tab2 = nullptr;
     const double* fb_master;
     double* fb_carry;
     int fb_count;
@@ -152,6 +156,7 @@ struct RevRunSpec {
     // and every step allows it, the runs use the tan butterfly (the table
     // carries each step's c^n) instead of (c, s) with phase_tab
     void* fb_applied = nullptr;
+    void* fb_applied2 = nullptr;  // optional second buffer (as RunSpec::phase_tab2)
     const double* fb_master = nullptr;
     double* fb_carry = nullptr;
     int fb_count = 0;
