@@ -130,15 +130,16 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-// Deterministic block reduction of NV doubles over NW waves (pairwise tree
-// over the waves).  'scratch' must hold NW*NV doubles and be free for use.
+// Deterministic block reduction of NV doubles over the first NW waves
+// (pairwise tree over the waves; later waves take part in the barrier only).
+// 'scratch' must hold NW*NV doubles and be free for use.
 template <int NV, int NW = 4>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* scratch, double* out)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
-    if (lane == 0)
+    if (lane == 0 && wave < NW)
 #pragma unroll
         for (int i = 0; i < NV; ++i) scratch[wave * NV + i] = v[i];
     __syncthreads();
@@ -155,6 +156,35 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* scra
             out[i] = t[0];
         }
     }
+}
+
+// The finishing sums load kFinishBatch strided partials per thread before
+// adding them in index order: the loads overlap instead of one L2 round trip
+// per partial (a 12288-partial gradient sum took 19 us as a dependent chain).
+constexpr int kFinishBatch = 8;
+
+// The first kThreads threads of the workgroup sum (any further threads take
+// part in the barrier only), so a sweep workgroup finishing a previous pass's
+// partials (SweepArgs::fin_out) gives the same bits as k_finish3.
+__device__ __forceinline__ void finish3_body(const double* __restrict__ partials, int count, double* __restrict__ out,
+                                             double* scratch)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    constexpr int bd = kThreads;
+    for (int i = threadIdx.x < kThreads ? static_cast<int>(threadIdx.x) : count; i < count; i += kFinishBatch * bd) {
+        double v[kFinishBatch][3];
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k) {
+            const int ik = i + k * bd;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[k][c] = ik < count ? partials[3 * ik + c] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] += v[k][c];
+    }
+    block_reduce_store<3>(acc, scratch, out);
 }
 
 // an LDS element by byte offset
@@ -239,6 +269,11 @@ struct SweepArgs {
     double* fb_carry;
     int fb_count;
     double fb_scale;
+    // fin_out non-null: the first workgroup sums fin_count partial triples of
+    // the previous pass (fin_partials) into fin_out, as k_finish3 does
+    const double* fin_partials;
+    int fin_count;
+    double* fin_out;
 };
 
 // entries of a phase table for n spins: E = 4m + c with E <= sum_{j<n} j^2
@@ -301,6 +336,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     constexpr int TB = P.TB, RB = P.RB, NR = 1 << RB, NP = P.NP, RR = P.R, TT = P.TT();
     constexpr int NW = (1 << TT) / 64;  // waves per workgroup
     static_assert(NS == 1 || NS == 2, "one or two states");
+    static_assert((1 << TT) >= kThreads, "a sweep workgroup holds the kThreads threads of a folded finish");
     static_assert(NS == 1 || (FLAGS & (F_MEASURE | F_PREFETCH)) == 0,
                   "two-state sweeps run without measurement or prefetch");
     static_assert((FLAGS & F_GRAD) == 0 || NS == 2, "the gradient pairs two states");
@@ -531,6 +567,11 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                  i += static_cast<int>(gridDim.x) << TT)
                 feedback_entry<V>(w[i], cr + i, nx + i, a.fb_scale);
         }
+    }
+    // the previous pass's observables, finished while the first tile loads
+    if (a.fin_out != nullptr && blockIdx.x == 0) {
+        __shared__ double fin_scratch[12];
+        finish3_body(a.fin_partials, a.fin_count, a.fin_out, fin_scratch);
     }
     for (; tile < a.ntiles; tile += gridDim.x) {
         uint64_t lt = tile;
@@ -839,32 +880,6 @@ __global__ __launch_bounds__(kThreads) void k_elementwise(V* __restrict__ psi, u
 }
 
 // sum 'count' triples of partials in a fixed order into out[0..2]
-// The finishing sums load kFinishBatch strided partials per thread before
-// adding them in index order: the loads overlap instead of one L2 round trip
-// per partial (a 12288-partial gradient sum took 19 us as a dependent chain).
-constexpr int kFinishBatch = 8;
-
-__device__ __forceinline__ void finish3_body(const double* __restrict__ partials, int count, double* __restrict__ out,
-                                             double* scratch)
-{
-    double acc[3] = {0.0, 0.0, 0.0};
-    const int bd = static_cast<int>(blockDim.x);
-    for (int i = threadIdx.x; i < count; i += kFinishBatch * bd) {
-        double v[kFinishBatch][3];
-#pragma unroll
-        for (int k = 0; k < kFinishBatch; ++k) {
-            const int ik = i + k * bd;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[k][c] = ik < count ? partials[3 * ik + c] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < kFinishBatch; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[c] += v[k][c];
-    }
-    block_reduce_store<3>(acc, scratch, out);
-}
-
 __global__ __launch_bounds__(kThreads) void k_finish3(const double* __restrict__ partials, int count,
                                                      double* __restrict__ out)
 {
@@ -1816,6 +1831,20 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
     }
     const bool tan = true;
     const bool fold = rs.phase_tab2 != nullptr;
+    // measured pass k writes partials region k & 1; the next pass of the run
+    // finishes it into the step's slot (the last one, k_finish3)
+    const size_t region_doubles = 3 * static_cast<size_t>(std::max(tun.grid, 4096));
+    struct Fin {
+        const double* part = nullptr;
+        int count = 0;
+        double* slot = nullptr;
+    } fin;
+    auto take_fin = [&](SweepArgs& b) {
+        b.fin_partials = fin.part;
+        b.fin_count = fin.count;
+        b.fin_out = fin.slot;
+        fin = Fin{};
+    };
     SweepArgs a{};
     a.tab = rs.phase_tab;
     a.partials = partials;
@@ -1854,6 +1883,7 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         if (two) {
             SweepArgs b = high_args(a, A, dim);
             coef(k, &b.bc, &b.bs);
+            take_fin(b);
             const int g = clamp_grid(b.ntiles, tun.grid);
             if (tun.timer) tun.timer->begin(st);
             // the next tile's loads go out before this tile's rotations (N=30: 3.242 -> 3.225 ms)
@@ -1889,16 +1919,24 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         }
         coef(k, &b.bc, &b.bs);
         if (post) coef(k + 1, &b.bc2, &b.bs2);
+        take_fin(b);
+        b.partials = partials + static_cast<size_t>(k & 1) * region_doubles;
         const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0);
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
         e = launch_ppass<V>(low, ltb, fl, group_tb(B), B.R, psi, b, g, st);
         if (e != hipSuccess) return e;
         if (tun.timer) tun.timer->end(st, low ? "fused_low_phase" : "fused_high_phase", pass_bytes);
-        if (slot) {
-            hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, partials, g, slot);
+        if (slot) fin = Fin{b.partials, g, slot};
+        if (fin.slot && !tun.fold_finish) {
+            hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, fin.part, fin.count, fin.slot);
             if ((e = hipGetLastError()) != hipSuccess) return e;
+            fin = Fin{};
         }
+    }
+    if (fin.slot) {
+        hipLaunchKernelGGL(k_finish3, dim3(1), dim3(kThreads), 0, st, fin.part, fin.count, fin.slot);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -2213,7 +2251,9 @@ SweepTuning default_tuning(int n, int precision)
     return t;
 }
 
-size_t partials_bytes(int grid) { return sizeof(double) * 3 * static_cast<size_t>(std::max(grid, 4096)); }
+// two regions: a fused run's measured pass k writes region k & 1 while the
+// next pass finishes region (k - 1) & 1
+size_t partials_bytes(int grid) { return 2 * sizeof(double) * 3 * static_cast<size_t>(std::max(grid, 4096)); }
 
 hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepSpec& spec,
                        const SweepTuning& tun, double* partials, hipStream_t stream)

@@ -71,6 +71,9 @@ struct SweepTuning {
     // LDS, phase table in global memory), two workgroups per CU, when its rows
     // keep at least 64 amplitudes (default_tuning: on)
     bool narrow_phase = false;
+    // fused runs: the pass after a measured phase pass finishes its partials
+    // (false: a k_finish3 kernel after each measured pass)
+    bool fold_finish = true;
     KernelTimer* timer = nullptr;
 };
 
