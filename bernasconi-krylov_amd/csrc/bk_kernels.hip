@@ -187,6 +187,23 @@ __device__ __forceinline__ void finish3_body(const double* __restrict__ partials
     block_reduce_store<3>(acc, scratch, out);
 }
 
+// 'count' doubles in a fixed order into *out by the first kThreads threads
+// (as finish3_body: a sweep workgroup folding it gives k_finish_sum's bits)
+__device__ __forceinline__ void finish_sum_body(const double* __restrict__ partials, int count, double* __restrict__ out,
+                                                double* scratch)
+{
+    double acc[1] = {0.0};
+    constexpr int bd = kThreads;
+    for (int i = threadIdx.x < kThreads ? static_cast<int>(threadIdx.x) : count; i < count; i += kFinishBatch * bd) {
+        double v[kFinishBatch];
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k) v[k] = i + k * bd < count ? partials[i + k * bd] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kFinishBatch; ++k) acc[0] += v[k];
+    }
+    block_reduce_store<1>(acc, scratch, out);
+}
+
 // an LDS element by byte offset
 template <typename V>
 __device__ __forceinline__ V& lds_at(unsigned char* base, uint32_t byte_off)
@@ -271,9 +288,11 @@ struct SweepArgs {
     double fb_scale;
     // fin_out non-null: the first workgroup sums fin_count partial triples of
     // the previous pass (fin_partials) into fin_out, as k_finish3 does
+    // (fin_single: fin_count doubles into one, as k_finish_sum does)
     const double* fin_partials;
     int fin_count;
     double* fin_out;
+    int fin_single;
 };
 
 // entries of a phase table for n spins: E = 4m + c with E <= sum_{j<n} j^2
@@ -571,7 +590,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     // the previous pass's observables, finished while the first tile loads
     if (a.fin_out != nullptr && blockIdx.x == 0) {
         __shared__ double fin_scratch[12];
-        finish3_body(a.fin_partials, a.fin_count, a.fin_out, fin_scratch);
+        if (a.fin_single) finish_sum_body(a.fin_partials, a.fin_count, a.fin_out, fin_scratch);
+        else finish3_body(a.fin_partials, a.fin_count, a.fin_out, fin_scratch);
     }
     for (; tile < a.ntiles; tile += gridDim.x) {
         uint64_t lt = tile;
@@ -1055,16 +1075,7 @@ __global__ __launch_bounds__(kThreads) void k_finish_sum(const double* __restric
                                                         double* __restrict__ out)
 {
     __shared__ double scratch[4];
-    double acc[1] = {0.0};
-    const int bd = static_cast<int>(blockDim.x);
-    for (int i = threadIdx.x; i < count; i += kFinishBatch * bd) {
-        double v[kFinishBatch];
-#pragma unroll
-        for (int k = 0; k < kFinishBatch; ++k) v[k] = i + k * bd < count ? partials[i + k * bd] : 0.0;
-#pragma unroll
-        for (int k = 0; k < kFinishBatch; ++k) acc[0] += v[k];
-    }
-    block_reduce_store<1>(acc, scratch, out);
+    finish_sum_body(partials, count, out, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -2083,6 +2094,17 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         e = timed("adjoint_open", [&] { return high_rev(false, B, b); });
     }
     if (e != hipSuccess) return e;
+    // step i's gradient sum: by the first workgroup of step i+1's A pass (two
+    // groups; it runs before the phase pass that rewrites region i & 1), else
+    // by k_finish_sum
+    int fin_step = -1;
+    auto finish_alone = [&]() -> hipError_t {
+        if (fin_step < 0) return hipSuccess;
+        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, region(fin_step), GS,
+                           rs.grad_slots[fin_step]);
+        fin_step = -1;
+        return hipGetLastError();
+    };
     for (int i = 0; i < K; ++i) {
         const bool even = (i & 1) == 0;
         if (two) {
@@ -2090,8 +2112,17 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
             coef(i, &b.bc, &b.bs);
             weights(b, i, even ? nL : nB, nA);
             b.partials = region(i) + offA;
+            if (fin_step >= 0 && tun.fold_finish) {
+                b.fin_partials = region(fin_step);
+                b.fin_count = GS;
+                b.fin_out = rs.grad_slots[fin_step];
+                b.fin_single = 1;
+                fin_step = -1;
+            }
+            if ((e = finish_alone()) != hipSuccess) return e;
             if ((e = timed("adjoint_high_a", [&] { return high_rev(false, A, b); })) != hipSuccess) return e;
         }
+        if ((e = finish_alone()) != hipSuccess) return e;
         // step i's table: P(-dt) times the exact c^n of its rotations, with
         // feedback; its own kernel, or (two table buffers) written by the
         // previous step's phase pass
@@ -2138,10 +2169,9 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         e = low ? timed("adjoint_low_phase", [&] { return low_rev(true, b); })
                 : timed("adjoint_high_phase", [&] { return high_rev(true, B, b); });
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kThreads), 0, st, region(i), GS, rs.grad_slots[i]);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        fin_step = i;
     }
-    return hipSuccess;
+    return finish_alone();
 }
 
 template <typename V, int FL, int R = kFusedRMin>
