@@ -75,6 +75,7 @@ struct Plan {
     size_t esz = 0;
     char* states = nullptr;
     char* applied = nullptr;
+    char* applied2 = nullptr;  // second table buffer: fused runs alternate per step
     double* carry = nullptr;
     double* master = nullptr;
     double *bc = nullptr, *bs = nullptr, *sd = nullptr, *re = nullptr, *im = nullptr;
@@ -137,11 +138,15 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
     };
     const int tb = bk::low_tile_bits(P.precision);
     const int htb = tb + 1;  // high-pass tile bits (2^14 c64, 2^13 c128)
-    auto feedback = [&](const Group& g, int first, int count) -> int {
+    // the tables of a block of states in table buffer 0 or 1
+    auto tabs = [&](const Group& g, int first, int buf) {
+        return (buf ? P.applied2 : P.applied) + (g.tab_off + static_cast<size_t>(first) * g.tab_count) * P.esz;
+    };
+    auto feedback = [&](const Group& g, int first, int count, int buf = 0) -> int {
         HIPCALL(bk::launch_batch_feedback(P.master + g.master_off,
                                           P.carry + 2 * (g.tab_off + static_cast<size_t>(first) * g.tab_count),
-                                          P.applied + (g.tab_off + static_cast<size_t>(first) * g.tab_count) * P.esz,
-                                          g.tab_count, count, P.sd + g.arr_off + first, P.precision, c->stream));
+                                          tabs(g, first, buf), g.tab_count, count, P.sd + g.arr_off + first,
+                                          P.precision, c->stream));
         ++nl;
         return BK_OK;
     };
@@ -189,7 +194,9 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
                 pa.hs = fg[0];
                 if ((rc = pass(pa, nullptr))) return rc;
             }
-            if ((rc = feedback(g, first, count))) return rc;
+            // step k's tables: buffer k & 1, set by step k-1's phase pass (the
+            // first by its own kernel)
+            if (k == 0 && (rc = feedback(g, first, count, 0))) return rc;
             const bool low = two ? (k & 1) != 0 : (k & 1) == 0;
             const bool last = k + 1 == nsteps;
             bk::BatchPass pp{};
@@ -198,7 +205,13 @@ int enqueue_run(bk_ctx* c, const Plan& P, int steps, const Tune& t, int64_t* lau
             pp.ecache_high = g.ecache_hi;
             pp.measure = last;
             pp.post = !last;
+            b.tab = tabs(g, first, k & 1);
+            b.tab_next = last ? nullptr : tabs(g, first, (k + 1) & 1);
+            b.fb_master = P.master + g.master_off;
+            b.fb_carry = P.carry + 2 * (g.tab_off + static_cast<size_t>(first) * g.tab_count);
+            b.fb_sd = P.sd + g.arr_off + first;
             if ((rc = pass(pp, last ? part_last : nullptr))) return rc;
+            b.tab_next = nullptr;
         }
         return BK_OK;
     };
@@ -322,8 +335,9 @@ static int bk_batch_run_impl(bk_ctx* c, const bk_instance* inst, int count, cons
     void* v = nullptr;
     if ((rc = arena(c, "states", P.total_states_bytes, &v))) return rc;
     P.states = static_cast<char*>(v);
-    if ((rc = arena(c, "applied", tab_off * P.esz, &v))) return rc;
+    if ((rc = arena(c, "applied", 2 * tab_off * P.esz, &v))) return rc;
     P.applied = static_cast<char*>(v);
+    P.applied2 = P.applied + tab_off * P.esz;
     if ((rc = arena(c, "carry", tab_off * 16, &v))) return rc;
     P.carry = static_cast<double*>(v);
     if ((rc = arena(c, "master", master_h.size() * 8, &v))) return rc;

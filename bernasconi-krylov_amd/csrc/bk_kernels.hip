@@ -286,6 +286,8 @@ struct SweepArgs {
     double* fb_carry;
     int fb_count;
     double fb_scale;
+    const double* fb_sd_arr;  // F_BATCH: a scale per state, fb_count entries per state
+    int fb_nstates;
     // fin_out non-null: the first workgroup sums fin_count partial triples of
     // the previous pass (fin_partials) into fin_out, as k_finish3 does
     // (fin_single: fin_count doubles into one, as k_finish_sum does)
@@ -577,14 +579,25 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     // the next step's error-feedback table, one entry per thread of the first
     // workgroups, while the first tile's loads are in flight (this pass reads
     // the other table buffer; the pass that read this one has finished)
-    if constexpr ((FLAGS & F_PHASE) != 0 && !BATCH) {
+    if constexpr ((FLAGS & F_PHASE) != 0) {
         if (a.fb_next != nullptr) {
             const double2* w = reinterpret_cast<const double2*>(a.fb_master);
             double2* cr = reinterpret_cast<double2*>(a.fb_carry);
             V* nx = static_cast<V*>(a.fb_next);
-            for (int i = static_cast<int>(blockIdx.x * (1u << TT) + t); i < a.fb_count;
-                 i += static_cast<int>(gridDim.x) << TT)
-                feedback_entry<V>(w[i], cr + i, nx + i, a.fb_scale);
+            if constexpr (!BATCH) {
+                for (int i = static_cast<int>(blockIdx.x * (1u << TT) + t); i < a.fb_count;
+                     i += static_cast<int>(gridDim.x) << TT)
+                    feedback_entry<V>(w[i], cr + i, nx + i, a.fb_scale);
+            } else {
+                // the tables of all states, k_phase_feedback_batch's indexing
+                const int64_t total = static_cast<int64_t>(a.fb_count) * a.fb_nstates;
+                for (int64_t k = static_cast<int64_t>(blockIdx.x) * (1 << TT) + t; k < total;
+                     k += static_cast<int64_t>(gridDim.x) << TT) {
+                    const int i = static_cast<int>(k / a.fb_count);
+                    const int m = static_cast<int>(k - static_cast<int64_t>(i) * a.fb_count);
+                    feedback_entry<V>(w[m], cr + k, nx + k, a.fb_sd_arr[i]);
+                }
+            }
         }
     }
     // the previous pass's observables, finished while the first tile loads
@@ -2201,6 +2214,14 @@ hipError_t batch_pass_impl(const BatchGroup& g, const BatchPass& bp, const Sweep
     a.e0_arr = g.e0;
     a.partials = partials;
     a.dst = g.psi_out;
+    if (g.tab_next && (bp.kind == BatchPass::kLowPhase || bp.kind == BatchPass::kHighPhase)) {
+        a.fb_next = g.tab_next;
+        a.fb_master = g.fb_master;
+        a.fb_carry = g.fb_carry;
+        a.fb_count = g.tab_stride;
+        a.fb_sd_arr = g.fb_sd;
+        a.fb_nstates = g.nstates;
+    }
     V* psi = static_cast<V*>(g.psi);
     const bool low = bp.kind == BatchPass::kLowPlain || bp.kind == BatchPass::kLowPhase;
     if (low) {

@@ -197,6 +197,13 @@ struct BatchGroup {
     const uint32_t* s0;       // per state (device)
     const uint32_t* e0;       // per state (device)
     bool tan_form;            // every state of the group uses the same butterfly form
+    // phase passes: when tab_next is non-null the pass also sets the next
+    // step's tables (tab_stride entries per state, the layout of tab) from
+    // fb_master (shared), the per-state carries fb_carry and scales fb_sd
+    void* tab_next = nullptr;
+    const double* fb_master = nullptr;
+    double* fb_carry = nullptr;
+    const double* fb_sd = nullptr;
 };
 // One Trotter step of every state in g: HIGH sweeps then the LOW sweep with the
 // phase; with partials non-null the LOW sweep also writes 3 partials per tile
