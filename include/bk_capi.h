@@ -441,6 +441,11 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  *              long (n = 26..28; smaller states stay in the last-level cache);
  *              0 = 2^14-amplitude tiles.  The state is the
  *              same bit for bit either way.
+ * "fold_finish" : 1 (default) = in a fused run, the pass after a measured
+ *              phase pass sums that pass's partials in its first workgroup (and
+ *              the reverse run's A pass the previous step's gradient partials);
+ *              0 = a finishing kernel after every measured pass.  Results are
+ *              the same bit for bit.
  * A key keeps its value for the context until set again; every other key takes
  * the defaults of each new state's n and precision. */
 int bk_set_tuning(bk_ctx* ctx, const char* key, int value);

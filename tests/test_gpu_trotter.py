@@ -270,6 +270,28 @@ def test_narrow_phase_group_bit_identical(bk, n):
         assert np.abs(ra[key] - rb[key]).max() <= tol, key
 
 
+@pytest.mark.parametrize("n,prec", [(22, "c64"), (26, "c64"), (24, "c128")])
+def test_folded_finish_bit_identical(bk, n, prec):
+    """A fused run finishes each measured pass's partials in the first
+    workgroup of the run's next pass (and k_finish3 only after its last pass);
+    the same arithmetic as a k_finish3 kernel after every measured pass, so
+    every recorded observable agrees bit for bit (one group at n = 22, two at
+    26 and 24)."""
+    precision = bk.BK_C64 if prec == "c64" else bk.BK_C128
+    _, s0, e0 = bk.make_trap(n, 11)
+    out = {}
+    for fold in (1, 0):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("fold_finish", fold)
+            out[fold] = eng.evolve(n, s0, 0.3, 0.02, 15, order=bk.BK_STRANG, precision=precision, e0=e0,
+                                   measure_every=1)
+        finally:
+            eng.close()
+    for key in ("p_less", "dh", "norm"):
+        assert np.array_equal(out[0][key], out[1][key]), key
+
+
 def test_tuning_key_keeps_other_defaults(bk):
     """bk_set_tuning pins only the key it sets: a later state of another n or
     precision takes that state's defaults for every other key (row bits,
