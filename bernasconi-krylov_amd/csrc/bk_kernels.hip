@@ -239,7 +239,7 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // to partials2: they belong to the next step.
 enum : int {
     F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32, F_BATCH = 64,
-    F_POSTROT = 128
+    F_POSTROT = 128, F_LDSTAB = 256
 };
 
 struct SweepArgs {
@@ -311,10 +311,21 @@ __host__ __device__ constexpr int phase_tab_count(int n)
 // 2^13-amplitude tiles (LOW, and the narrow phase-carrying high group) keep
 // the table in global memory, where the extra LDS would halve their
 // workgroups per CU.  Batched passes keep per-state tables in global memory.
+// F_LDSTAB: a 2^13-amplitude complex64 phase pass of n <= kLdsTabMaxN keeps
+// the table in LDS too: its 1929 entries at most (15.4 KB) beside the 64 KB
+// exchange still leave two workgroups per CU.
+constexpr int kLdsTabMaxN = 29;
 template <typename V, bool LOW, int TB, int FLAGS>
 constexpr bool lds_phase_table()
 {
-    return TB >= 14 && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 && sizeof(V) == 8;
+    return (TB >= 14 || (FLAGS & F_LDSTAB) != 0) && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 &&
+           sizeof(V) == 8;
+}
+// entries the LDS table holds
+template <int TB, int FLAGS>
+constexpr int lds_phase_table_cap()
+{
+    return TB >= 14 ? phase_tab_count(32) : phase_tab_count(kLdsTabMaxN);
 }
 
 // Error-feedback phase table: applied = round(sd * master * carry),
@@ -387,8 +398,10 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     constexpr bool LDS_TAB = lds_phase_table<V, P.low, TB, FLAGS>();
     constexpr int XCH = NP > 1 ? 1 << TB : 0;
     constexpr int SCRATCH = (3 * NW * 8 + static_cast<int>(sizeof(V)) - 1) / static_cast<int>(sizeof(V));
-    constexpr int LDS_ELEMS = XCH + (LDS_TAB ? phase_tab_count(32) : 0) > SCRATCH
-                                  ? XCH + (LDS_TAB ? phase_tab_count(32) : 0)
+    constexpr int TABCAP = LDS_TAB ? lds_phase_table_cap<TB, FLAGS>() : 0;
+    static_assert((FLAGS & F_LDSTAB) == 0 || TB == 13, "the 2^13-amplitude tiles' LDS table");
+    constexpr int LDS_ELEMS = XCH + TABCAP > SCRATCH
+                                  ? XCH + TABCAP
                                   : SCRATCH;
     __shared__ __align__(16) V lds_buf[LDS_ELEMS];
     unsigned char* smem_raw = reinterpret_cast<unsigned char*>(lds_buf);
@@ -1661,7 +1674,7 @@ hipError_t launch_low_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 template <typename V, int FL>
 hipError_t launch_low_run(int ltb, V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
-    if constexpr (std::is_same_v<V, float2>) {
+    if constexpr (std::is_same_v<V, float2> && (FL & F_LDSTAB) == 0) {
         if (ltb == kWideLowTB) return launch_low_fl<V, FL, kWideLowTB>(psi, a, grid, st);
     }
     if (ltb != Cfg<V>::TB) return hipErrorInvalidValue;
@@ -1729,8 +1742,25 @@ hipError_t launch_ppass(bool low, int ltb, int fl, int htb, int rbits, V* psi, c
     case B | F_MEASURE | F_POSTROT:
         return low ? launch_low_run<V, B | F_MEASURE | F_POSTROT>(ltb, psi, a, grid, st)
                    : launch_highp<V, B | F_MEASURE | F_POSTROT>(htb, rbits, psi, a, grid, st);
-    default: return hipErrorInvalidValue;
     }
+    if constexpr (std::is_same_v<V, float2>) {
+        // 2^13-amplitude tiles with the phase table in LDS (n <= kLdsTabMaxN)
+        constexpr int BL = B | F_LDSTAB;
+        switch (fl) {
+        case BL:
+            return low ? launch_low_run<V, BL>(ltb, psi, a, grid, st) : launch_highn_r<V, BL>(rbits, psi, a, grid, st);
+        case BL | F_MEASURE:
+            return low ? launch_low_run<V, BL | F_MEASURE>(ltb, psi, a, grid, st)
+                       : launch_highn_r<V, BL | F_MEASURE>(rbits, psi, a, grid, st);
+        case BL | F_POSTROT:
+            return low ? launch_low_run<V, BL | F_POSTROT>(ltb, psi, a, grid, st)
+                       : launch_highn_r<V, BL | F_POSTROT>(rbits, psi, a, grid, st);
+        case BL | F_MEASURE | F_POSTROT:
+            return low ? launch_low_run<V, BL | F_MEASURE | F_POSTROT>(ltb, psi, a, grid, st)
+                       : launch_highn_r<V, BL | F_MEASURE | F_POSTROT>(rbits, psi, a, grid, st);
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 // The high-qubit groups of a fused run whose LOW pass covers qubits [0, ltb)
@@ -1945,7 +1975,12 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         if (post) coef(k + 1, &b.bc2, &b.bs2);
         take_fin(b);
         b.partials = partials + static_cast<size_t>(k & 1) * region_doubles;
-        const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0);
+        // complex64 2^13-amplitude tiles (the default LOW tile, the narrow
+        // group) take the phase table into LDS while it fits
+        const bool ldstab = std::is_same_v<V, float2> && tun.lds_tab13 && n <= kLdsTabMaxN &&
+                            (low ? ltb == kNarrowTB : group_tb(B) == kNarrowTB);
+        const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0) |
+                       (ldstab ? F_LDSTAB : 0);
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
         e = launch_ppass<V>(low, ltb, fl, group_tb(B), B.R, psi, b, g, st);
@@ -1988,7 +2023,9 @@ hipError_t launch_high_rev_ppass_r(V* phi, int rbits, const SweepArgs& a, int gr
 template <typename V, int FL = kRevPass>
 hipError_t launch_high_rev_ppass(V* phi, int htb, int rbits, const SweepArgs& a, int grid, hipStream_t st)
 {
-    if (htb == Cfg<V>::HTB) return launch_high_rev_ppass_r<V, FL>(phi, rbits, a, grid, st);
+    if constexpr ((FL & F_LDSTAB) == 0) {
+        if (htb == Cfg<V>::HTB) return launch_high_rev_ppass_r<V, FL>(phi, rbits, a, grid, st);
+    }
     if constexpr (std::is_same_v<V, float2>) {
         if (htb == kNarrowTB) return launch_high_rev_ppass_r<V, FL, kNarrowRMin, kNarrowTB>(phi, rbits, a, grid, st);
     }
@@ -2074,8 +2111,10 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         if (r == hipSuccess && tun.timer) tun.timer->end(st, name, pass_bytes);
         return r;
     };
+    const bool ldstab = std::is_same_v<V, float2> && tun.lds_tab13 && n <= kLdsTabMaxN;
     auto low_rev = [&](bool phase, const SweepArgs& b) -> hipError_t {
         if constexpr (std::is_same_v<V, float2>) {
+            if (tan && phase && ldstab) return launch_low_rev<V, kRevPass | F_TAN | F_LDSTAB>(phi, b, gL, st);
             if (tan) return phase ? launch_low_rev<V, kRevPass | F_TAN>(phi, b, gL, st)
                                   : launch_low_rev<V, F_GRAD | F_TAN>(phi, b, gL, st);
         }
@@ -2087,6 +2126,8 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
         const int R = hs.R, htb = group_tb(hs);
         if (!phase && htb != C::HTB) return hipErrorInvalidValue;
         if constexpr (std::is_same_v<V, float2>) {
+            if (tan && phase && ldstab && htb == kNarrowTB)
+                return launch_high_rev_ppass<V, kRevPass | F_TAN | F_LDSTAB>(phi, htb, R, b, gH, st);
             if (tan) return phase ? launch_high_rev_ppass<V, kRevPass | F_TAN>(phi, htb, R, b, gH, st)
                                   : launch_high_rev<V, F_GRAD | F_TAN>(phi, R, b, gH, st);
         }
@@ -2291,6 +2332,11 @@ SweepTuning default_tuning(int n, int precision)
     // 0.460 -> 0.436 ms), N=27 0.485 -> 0.465, N=28 0.918 -> 0.899; the
     // adjoint's reverse B phase pass at N=26 0.513 -> 0.460 ms
     t.narrow_phase = true;
+    // LDS phase table for the 2^13-amplitude phase passes: N=26 complex64 step
+    // 0.4255 -> 0.4116 ms, adjoint forward 0.531 -> 0.520 s and backward
+    // 1.145 -> 1.134 s; N=27 and N=24 unchanged, N=25, 28-29 and 20 0.5-3 %
+    // slower with it
+    t.lds_tab13 = precision == kC64 && (n == 26 || n == 27);
     t.prefetch_low = false;
     t.prefetch_high = false;
     // page-balancing granule of the high groups' planner, in amplitudes: 2^20

@@ -74,6 +74,10 @@ struct SweepTuning {
     // fused runs: the pass after a measured phase pass finishes its partials
     // (false: a k_finish3 kernel after each measured pass)
     bool fold_finish = true;
+    // complex64 phase passes on 2^13-amplitude tiles take the phase table
+    // into LDS for n <= 29 (it fits beside the 64 KB exchange at two
+    // workgroups per CU; default_tuning: n = 26..27)
+    bool lds_tab13 = false;
     KernelTimer* timer = nullptr;
 };
 
