@@ -76,10 +76,13 @@ __device__ __forceinline__ void store_amp(V* p, const V& v)
 }
 
 // Amplitude load of a sweep.  NT: a nontemporal load (streamed through the
-// caches without keeping the line), which leaves the vector L1 to the
-// phase-table gathers of one-state passes that read the table from global
-// memory (N=30 LOW phase pass 3.645 -> 3.612 ms; two-state passes are slower
-// with it: 0.560 -> 0.575 ms at N=26).
+// caches without keeping the line).  Measured per pass kind (same box, same
+// results): faster for the LOW passes (N=30 plain 3.32 -> 3.16 ms, phase pass
+// with the table in global memory 3.645 -> 3.612 ms), the N=26 narrow phase
+// pass with its table in LDS (0.221 -> 0.202 ms) and every complex128 pass
+// (N=28 step 3.40 -> 3.37 ms); slower for the complex64 plain high-group pass
+// (N=30 3.220 -> 3.230 ms, N=26 0.188 -> 0.196 ms) and the two-state passes
+// (N=26 LOW phase 0.560 -> 0.575 ms), which keep cached loads.
 template <bool NT, typename V>
 __device__ __forceinline__ V load_amp(const V* p)
 {
@@ -379,7 +382,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     static_assert(!BATCH || (NS == 1 && (FLAGS & F_PREFETCH) == 0), "batched sweeps: one state per tile, no prefetch");
     constexpr bool LOW = P.low;
     constexpr bool NTSTORE = !BATCH && (LOW || (FLAGS & F_PHASE) == 0);
-    constexpr bool NTLOAD = !BATCH && NS == 1 && (FLAGS & F_PHASE) != 0 && !lds_phase_table<V, P.low, TB, FLAGS>();
+    // nontemporal loads in one-state passes, except the complex64 plain high-group pass
+    constexpr bool NTLOAD = !BATCH && NS == 1 && !(sizeof(V) == 8 && !LOW && (FLAGS & F_PHASE) == 0);
     V* sp[NS];
     V* dp[NS];
     sp[0] = psi;
