@@ -317,12 +317,15 @@ __host__ __device__ constexpr int phase_tab_count(int n)
 // F_LDSTAB: a 2^13-amplitude complex64 phase pass of n <= kLdsTabMaxN keeps
 // the table in LDS too: its 1929 entries at most (15.4 KB) beside the 64 KB
 // exchange still leave two workgroups per CU.
-constexpr int kLdsTabMaxN = 29;
+// Complex128 high groups (2^13 amplitudes, 128 KB of exchange, one workgroup
+// per CU) fit the table's 1929 x 16 B (30.9 KB) beside it up to the same n;
+// their rows there are 2^4 .. 2^8 amplitudes.
+constexpr int kLdsTabMaxN = 29, kLdsTab128RMin = 4, kLdsTab128RMax = 8;
 template <typename V, bool LOW, int TB, int FLAGS>
 constexpr bool lds_phase_table()
 {
-    return (TB >= 14 || (FLAGS & F_LDSTAB) != 0) && (FLAGS & F_BATCH) == 0 && (FLAGS & F_PHASE) != 0 &&
-           sizeof(V) == 8;
+    return ((TB >= 14 && sizeof(V) == 8) || (FLAGS & F_LDSTAB) != 0) && (FLAGS & F_BATCH) == 0 &&
+           (FLAGS & F_PHASE) != 0;
 }
 // entries the LDS table holds
 template <int TB, int FLAGS>
@@ -1698,7 +1701,7 @@ hipError_t launch_high_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 template <typename V, int FL, int R = kFusedRMin>
 hipError_t launch_highp_r(int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
-    if constexpr (R > kFusedRMax) {
+    if constexpr (R > ((FL & F_LDSTAB) ? kLdsTab128RMax : kFusedRMax)) {
         return hipErrorInvalidValue;
     } else {
         if (rbits == R) return launch_high_fl<V, R, FL>(psi, a, grid, st);
@@ -1746,6 +1749,18 @@ hipError_t launch_ppass(bool low, int ltb, int fl, int htb, int rbits, V* psi, c
     case B | F_MEASURE | F_POSTROT:
         return low ? launch_low_run<V, B | F_MEASURE | F_POSTROT>(ltb, psi, a, grid, st)
                    : launch_highp<V, B | F_MEASURE | F_POSTROT>(htb, rbits, psi, a, grid, st);
+    }
+    if constexpr (std::is_same_v<V, double2>) {
+        // complex128 high groups (2^13-amplitude tiles) with the phase table in LDS
+        constexpr int BL = B | F_LDSTAB;
+        if (low) return hipErrorInvalidValue;
+        switch (fl) {
+        case BL: return launch_highp_r<V, BL, kLdsTab128RMin>(rbits, psi, a, grid, st);
+        case BL | F_MEASURE: return launch_highp_r<V, BL | F_MEASURE, kLdsTab128RMin>(rbits, psi, a, grid, st);
+        case BL | F_POSTROT: return launch_highp_r<V, BL | F_POSTROT, kLdsTab128RMin>(rbits, psi, a, grid, st);
+        case BL | F_MEASURE | F_POSTROT:
+            return launch_highp_r<V, BL | F_MEASURE | F_POSTROT, kLdsTab128RMin>(rbits, psi, a, grid, st);
+        }
     }
     if constexpr (std::is_same_v<V, float2>) {
         // 2^13-amplitude tiles with the phase table in LDS (n <= kLdsTabMaxN)
@@ -1981,8 +1996,9 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         b.partials = partials + static_cast<size_t>(k & 1) * region_doubles;
         // complex64 2^13-amplitude tiles (the default LOW tile, the narrow
         // group) take the phase table into LDS while it fits
-        const bool ldstab = std::is_same_v<V, float2> && tun.lds_tab13 && n <= kLdsTabMaxN &&
-                            (low ? ltb == kNarrowTB : group_tb(B) == kNarrowTB);
+        const bool ldstab = tun.lds_tab13 && n <= kLdsTabMaxN &&
+                            (std::is_same_v<V, float2> ? (low ? ltb == kNarrowTB : group_tb(B) == kNarrowTB)
+                                                       : (!low && group_tb(B) == Cfg<V>::HTB));
         const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0) |
                        (ldstab ? F_LDSTAB : 0);
         const int g = clamp_grid(b.ntiles, tun.grid);
@@ -2340,7 +2356,9 @@ SweepTuning default_tuning(int n, int precision)
     // 0.4255 -> 0.4116 ms, adjoint forward 0.531 -> 0.520 s and backward
     // 1.145 -> 1.134 s; N=27 and N=24 unchanged, N=25, 28-29 and 20 0.5-3 %
     // slower with it
-    t.lds_tab13 = precision == kC64 && (n == 26 || n == 27);
+    // complex128 high phase passes (one workgroup per CU either way): N=22-28
+    // 0.6-1.0 % faster per step with the table in LDS, N=29 unchanged
+    t.lds_tab13 = precision == kC64 ? (n == 26 || n == 27) : (n >= 22 && n <= 28);
     t.prefetch_low = false;
     t.prefetch_high = false;
     // page-balancing granule of the high groups' planner, in amplitudes: 2^20

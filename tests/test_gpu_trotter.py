@@ -270,10 +270,11 @@ def test_narrow_phase_group_bit_identical(bk, n):
         assert np.abs(ra[key] - rb[key]).max() <= tol, key
 
 
-@pytest.mark.parametrize("n", [22, 26, 28])
-def test_lds_phase_table_bit_identical(bk, n):
-    """complex64 phase passes on 2^13-amplitude tiles with the phase table in
-    LDS (default at n = 26..27) against the same passes reading it from
+@pytest.mark.parametrize("n,prec", [(22, "c64"), (26, "c64"), (28, "c64"), (24, "c128")])
+def test_lds_phase_table_bit_identical(bk, n, prec):
+    """Phase passes on 2^13-amplitude tiles with the phase table in LDS
+    (complex64 LOW and narrow passes, default at n = 26..27; complex128 high
+    groups, default at n = 22..28) against the same passes reading it from
     global memory: the same entries multiply the same amplitudes, so the
     state and every observable agree bit for bit."""
     _, s0, e0 = bk.make_trap(n, 13)
@@ -282,12 +283,14 @@ def test_lds_phase_table_bit_identical(bk, n):
         eng = bk.Engine(0)
         try:
             eng.set_tuning("lds_tab13", lds)
-            r = eng.evolve(n, s0, 0.35, 0.02, 9, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0, measure_every=1)
+            r = eng.evolve(n, s0, 0.35, 0.02, 9, order=bk.BK_STRANG,
+                           precision=bk.BK_C64 if prec == "c64" else bk.BK_C128, e0=e0, measure_every=1)
             out[lds] = (r, eng.download())
         finally:
             eng.close()
     (ra, pa), (rb, pb) = out[0], out[1]
-    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    assert np.array_equal(pa.view(np.uint64 if prec == "c128" else np.uint32),
+                          pb.view(np.uint64 if prec == "c128" else np.uint32))
     for key in ("p_less", "dh", "norm"):
         assert np.array_equal(ra[key], rb[key]), key
 
