@@ -242,7 +242,7 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // to partials2: they belong to the next step.
 enum : int {
     F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32, F_BATCH = 64,
-    F_POSTROT = 128, F_LDSTAB = 256
+    F_POSTROT = 128, F_LDSTAB = 256, F_LDSTABP = 512
 };
 
 struct SweepArgs {
@@ -320,18 +320,23 @@ __host__ __device__ constexpr int phase_tab_count(int n)
 // Complex128 high groups (2^13 amplitudes, 128 KB of exchange, one workgroup
 // per CU) fit the table's 1929 x 16 B (30.9 KB) beside it up to the same n;
 // their rows there are 2^4 .. 2^8 amplitudes.
-constexpr int kLdsTabMaxN = 29, kLdsTab128RMin = 4, kLdsTab128RMax = 8;
+// F_LDSTABP (complex128 high groups, n >= 30): the first kLdsTabPart entries
+// go to LDS (E < 4 kLdsTabPart + c_n, every state but those within a few
+// domain walls of the two aligned ones); a code beyond them is read from
+// global memory by the lanes that hold one.
+constexpr int kLdsTabMaxN = 29, kLdsTab128RMin = 4, kLdsTab128RMax = 8, kLdsTabPart = 2032;
+constexpr int kLdsTabPartRMin = 3, kLdsTabPartRMax = 5;  // complex128 rows at n = 30..32
 template <typename V, bool LOW, int TB, int FLAGS>
 constexpr bool lds_phase_table()
 {
-    return ((TB >= 14 && sizeof(V) == 8) || (FLAGS & F_LDSTAB) != 0) && (FLAGS & F_BATCH) == 0 &&
+    return ((TB >= 14 && sizeof(V) == 8) || (FLAGS & (F_LDSTAB | F_LDSTABP)) != 0) && (FLAGS & F_BATCH) == 0 &&
            (FLAGS & F_PHASE) != 0;
 }
 // entries the LDS table holds
 template <int TB, int FLAGS>
 constexpr int lds_phase_table_cap()
 {
-    return TB >= 14 ? phase_tab_count(32) : phase_tab_count(kLdsTabMaxN);
+    return (FLAGS & F_LDSTABP) != 0 ? kLdsTabPart : TB >= 14 ? phase_tab_count(32) : phase_tab_count(kLdsTabMaxN);
 }
 
 // Error-feedback phase table: applied = round(sd * master * carry),
@@ -406,7 +411,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     constexpr int XCH = NP > 1 ? 1 << TB : 0;
     constexpr int SCRATCH = (3 * NW * 8 + static_cast<int>(sizeof(V)) - 1) / static_cast<int>(sizeof(V));
     constexpr int TABCAP = LDS_TAB ? lds_phase_table_cap<TB, FLAGS>() : 0;
-    static_assert((FLAGS & F_LDSTAB) == 0 || TB == 13, "the 2^13-amplitude tiles' LDS table");
+    static_assert((FLAGS & (F_LDSTAB | F_LDSTABP)) == 0 || TB == 13, "the 2^13-amplitude tiles' LDS table");
+    static_assert((FLAGS & F_LDSTABP) == 0 || sizeof(V) == 16, "the partial LDS table: complex128 high groups");
     constexpr int LDS_ELEMS = XCH + TABCAP > SCRATCH
                                   ? XCH + TABCAP
                                   : SCRATCH;
@@ -460,7 +466,7 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     if constexpr (LDS_TAB) {
         V* tl = lds + XCH;
         const V* gt = static_cast<const V*>(a.tab);
-        const int cnt = phase_tab_count(a.n);
+        const int cnt = (FLAGS & F_LDSTABP) != 0 ? min(phase_tab_count(a.n), TABCAP) : phase_tab_count(a.n);
         for (int i = static_cast<int>(t); i < cnt; i += 1 << TT) tl[i] = gt[i];
         __syncthreads();
         ltab = tl;
@@ -745,7 +751,13 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
                 }
                 if constexpr ((FLAGS & F_PHASE) != 0) {
                     // E = 4m + c_n: one entry per m; the table carries the step's scale
-                    const V ph = LDS_TAB ? ltab[mcode] : tab[mcode];
+                    V ph;
+                    if constexpr ((FLAGS & F_LDSTABP) != 0) {
+                        if (mcode < static_cast<uint32_t>(TABCAP)) ph = ltab[mcode];
+                        else ph = tab[mcode];
+                    } else {
+                        ph = LDS_TAB ? ltab[mcode] : tab[mcode];
+                    }
                     const R pr = ph.x, pi = ph.y;
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
@@ -1701,7 +1713,7 @@ hipError_t launch_high_fl(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 template <typename V, int FL, int R = kFusedRMin>
 hipError_t launch_highp_r(int rbits, V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
-    if constexpr (R > ((FL & F_LDSTAB) ? kLdsTab128RMax : kFusedRMax)) {
+    if constexpr (R > ((FL & F_LDSTAB) ? kLdsTab128RMax : (FL & F_LDSTABP) ? kLdsTabPartRMax : kFusedRMax)) {
         return hipErrorInvalidValue;
     } else {
         if (rbits == R) return launch_high_fl<V, R, FL>(psi, a, grid, st);
@@ -1760,6 +1772,14 @@ hipError_t launch_ppass(bool low, int ltb, int fl, int htb, int rbits, V* psi, c
         case BL | F_POSTROT: return launch_highp_r<V, BL | F_POSTROT, kLdsTab128RMin>(rbits, psi, a, grid, st);
         case BL | F_MEASURE | F_POSTROT:
             return launch_highp_r<V, BL | F_MEASURE | F_POSTROT, kLdsTab128RMin>(rbits, psi, a, grid, st);
+        }
+        constexpr int BP = B | F_LDSTABP;
+        switch (fl) {
+        case BP: return launch_highp_r<V, BP, kLdsTabPartRMin>(rbits, psi, a, grid, st);
+        case BP | F_MEASURE: return launch_highp_r<V, BP | F_MEASURE, kLdsTabPartRMin>(rbits, psi, a, grid, st);
+        case BP | F_POSTROT: return launch_highp_r<V, BP | F_POSTROT, kLdsTabPartRMin>(rbits, psi, a, grid, st);
+        case BP | F_MEASURE | F_POSTROT:
+            return launch_highp_r<V, BP | F_MEASURE | F_POSTROT, kLdsTabPartRMin>(rbits, psi, a, grid, st);
         }
     }
     if constexpr (std::is_same_v<V, float2>) {
@@ -1999,8 +2019,11 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
         const bool ldstab = tun.lds_tab13 && n <= kLdsTabMaxN &&
                             (std::is_same_v<V, float2> ? (low ? ltb == kNarrowTB : group_tb(B) == kNarrowTB)
                                                        : (!low && group_tb(B) == Cfg<V>::HTB));
+        // complex128 high groups beyond kLdsTabMaxN: the table's first entries in LDS
+        const bool ldstabp = std::is_same_v<V, double2> && tun.lds_tab13 && n > kLdsTabMaxN && !low &&
+                             group_tb(B) == Cfg<V>::HTB;
         const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0) |
-                       (ldstab ? F_LDSTAB : 0);
+                       (ldstab ? F_LDSTAB : 0) | (ldstabp ? F_LDSTABP : 0);
         const int g = clamp_grid(b.ntiles, tun.grid);
         if (tun.timer) tun.timer->begin(st);
         e = launch_ppass<V>(low, ltb, fl, group_tb(B), B.R, psi, b, g, st);
@@ -2357,8 +2380,9 @@ SweepTuning default_tuning(int n, int precision)
     // 1.145 -> 1.134 s; N=27 and N=24 unchanged, N=25, 28-29 and 20 0.5-3 %
     // slower with it
     // complex128 high phase passes (one workgroup per CU either way): N=22-28
-    // 0.6-1.0 % faster per step with the table in LDS, N=29 unchanged
-    t.lds_tab13 = precision == kC64 ? (n == 26 || n == 27) : (n >= 22 && n <= 28);
+    // 0.6-1.0 % faster per step with the table in LDS, N=29 unchanged; N=30-32
+    // with its first 2032 entries in LDS the B phase pass runs 1-2 % faster
+    t.lds_tab13 = precision == kC64 ? (n == 26 || n == 27) : n >= 22;
     t.prefetch_low = false;
     t.prefetch_high = false;
     // page-balancing granule of the high groups' planner, in amplitudes: 2^20

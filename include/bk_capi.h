@@ -443,10 +443,11 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  *              same bit for bit either way.
  * "lds_tab13" : phase passes on 2^13-amplitude tiles (complex64: the LOW
  *              tile and the narrow phase-carrying group; complex128: the high
- *              groups) read the phase table from LDS (n <= 29; default on at
- *              complex64 n = 26..27 and complex128 n = 22..28, where it
- *              measured faster) instead of global memory; results are the same
- *              bit for bit.
+ *              groups) read the phase table from LDS (complex64 n <= 29;
+ *              complex128 the whole table to n = 29, its first 2032 entries
+ *              beyond; default on at complex64 n = 26..27 and complex128
+ *              n >= 22) instead of global memory; results are the same bit for
+ *              bit.
  * "fold_finish" : 1 (default) = in a fused run, the pass after a measured
  *              phase pass sums that pass's partials in its first workgroup (and
  *              the reverse run's A pass the previous step's gradient partials);

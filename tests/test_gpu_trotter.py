@@ -270,11 +270,12 @@ def test_narrow_phase_group_bit_identical(bk, n):
         assert np.abs(ra[key] - rb[key]).max() <= tol, key
 
 
-@pytest.mark.parametrize("n,prec", [(22, "c64"), (26, "c64"), (28, "c64"), (24, "c128")])
+@pytest.mark.parametrize("n,prec", [(22, "c64"), (26, "c64"), (28, "c64"), (24, "c128"), (30, "c128")])
 def test_lds_phase_table_bit_identical(bk, n, prec):
     """Phase passes on 2^13-amplitude tiles with the phase table in LDS
     (complex64 LOW and narrow passes, default at n = 26..27; complex128 high
-    groups, default at n = 22..28) against the same passes reading it from
+    groups, default from n = 22, only the first 2032 entries beyond n = 29)
+    against the same passes reading it from
     global memory: the same entries multiply the same amplitudes, so the
     state and every observable agree bit for bit."""
     _, s0, e0 = bk.make_trap(n, 13)
@@ -283,14 +284,16 @@ def test_lds_phase_table_bit_identical(bk, n, prec):
         eng = bk.Engine(0)
         try:
             eng.set_tuning("lds_tab13", lds)
-            r = eng.evolve(n, s0, 0.35, 0.02, 9, order=bk.BK_STRANG,
+            r = eng.evolve(n, s0, 0.35, 0.02, 9 if n < 30 else 4, order=bk.BK_STRANG,
                            precision=bk.BK_C64 if prec == "c64" else bk.BK_C128, e0=e0, measure_every=1)
-            out[lds] = (r, eng.download())
+            # beyond n = 28 the states are compared through their observables
+            out[lds] = (r, eng.download() if n <= 28 else None)
         finally:
             eng.close()
     (ra, pa), (rb, pb) = out[0], out[1]
-    assert np.array_equal(pa.view(np.uint64 if prec == "c128" else np.uint32),
-                          pb.view(np.uint64 if prec == "c128" else np.uint32))
+    if pa is not None:
+        assert np.array_equal(pa.view(np.uint64 if prec == "c128" else np.uint32),
+                              pb.view(np.uint64 if prec == "c128" else np.uint32))
     for key in ("p_less", "dh", "norm"):
         assert np.array_equal(ra[key], rb[key]), key
 
