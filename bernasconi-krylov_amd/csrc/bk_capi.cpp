@@ -178,6 +178,9 @@ int get_fb_tab(bk_ctx* c, int n, int precision, double dtp, bk_ctx::FbTab** out)
     auto it = c->fb_tabs.find(key);
     if (it == c->fb_tabs.end()) {
         if (c->fb_tabs.size() >= 16) {  // bound the cache (a cleared table restarts its carry)
+            // queued passes may still read these tables; each call fetches one
+            // table, so none handed out earlier in this call is freed here
+            HIPCALL(hipStreamSynchronize(c->stream));
             for (auto& kv : c->fb_tabs) {
                 kv.second.master.release();
                 kv.second.carry.release();

@@ -390,8 +390,13 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     static_assert(!BATCH || (NS == 1 && (FLAGS & F_PREFETCH) == 0), "batched sweeps: one state per tile, no prefetch");
     constexpr bool LOW = P.low;
     constexpr bool NTSTORE = !BATCH && (LOW || (FLAGS & F_PHASE) == 0);
-    // nontemporal loads in one-state passes, except the complex64 plain high-group pass
-    constexpr bool NTLOAD = !BATCH && NS == 1 && !(sizeof(V) == 8 && !LOW && (FLAGS & F_PHASE) == 0);
+    // nontemporal loads in one-state passes, except the complex64 plain high-group
+    // pass, and in complex64 two-state passes but the LOW phase pass.  Two-state
+    // passes at N=26: the high-group phase pass runs 0.447 -> 0.425 ms when the
+    // plain pass before it loads nontemporally too (0.447 with its own loads
+    // alone), the plain pass 0.397 -> 0.402 ms, the LOW phase pass 0.516 -> 0.519
+    constexpr bool NTLOAD = !BATCH && (NS == 1 ? !(sizeof(V) == 8 && !LOW && (FLAGS & F_PHASE) == 0)
+                                               : sizeof(V) == 8 && !(LOW && (FLAGS & F_PHASE) != 0));
     V* sp[NS];
     V* dp[NS];
     sp[0] = psi;
