@@ -98,6 +98,8 @@ int apply_tuning_key(bk_ctx* c, bk::SweepTuning& t, const std::string& k, int va
         t.wide_low = value != 0;
     } else if (k == "lds_tab13") {
         t.lds_tab13 = value != 0;
+    } else if (k == "nt_plain") {
+        t.nt_plain = value != 0;
     } else if (k == "fold_finish") {
         t.fold_finish = value != 0;
     } else if (k == "narrow_phase") {

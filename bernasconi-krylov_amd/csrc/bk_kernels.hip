@@ -242,7 +242,7 @@ __device__ __forceinline__ uint32_t swz(uint32_t u)
 // to partials2: they belong to the next step.
 enum : int {
     F_PHASE = 1, F_MEASURE = 2, F_TAN = 4, F_ECACHE = 8, F_PREFETCH = 16, F_GRAD = 32, F_BATCH = 64,
-    F_POSTROT = 128, F_LDSTAB = 256, F_LDSTABP = 512
+    F_POSTROT = 128, F_LDSTAB = 256, F_LDSTABP = 512, F_NTLOAD = 1024
 };
 
 struct SweepArgs {
@@ -395,7 +395,8 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
     // passes at N=26: the high-group phase pass runs 0.447 -> 0.425 ms when the
     // plain pass before it loads nontemporally too (0.447 with its own loads
     // alone), the plain pass 0.397 -> 0.402 ms, the LOW phase pass 0.516 -> 0.519
-    constexpr bool NTLOAD = !BATCH && (NS == 1 ? !(sizeof(V) == 8 && !LOW && (FLAGS & F_PHASE) == 0)
+    // F_NTLOAD: the complex64 plain high-group pass too (fused runs at n >= 27)
+    constexpr bool NTLOAD = !BATCH && (NS == 1 ? !(sizeof(V) == 8 && !LOW && (FLAGS & (F_PHASE | F_NTLOAD)) == 0)
                                                : sizeof(V) == 8 && !(LOW && (FLAGS & F_PHASE) != 0));
     V* sp[NS];
     V* dp[NS];
@@ -1327,30 +1328,36 @@ hipError_t launch_low_dyn(int fl, V* psi, const SweepArgs& a, int grid, hipStrea
     }
 }
 
-template <typename V, int R, bool TAN, bool PF>
+template <typename V, int R, bool TAN, bool PF, bool NT = false>
 hipError_t launch_high_r(V* psi, const SweepArgs& a, int grid, hipStream_t st)
 {
     using C = Cfg<V>;
     using PH = HighPlan<C::HTB, C::HRB, R>;
-    auto k = k_sweep<V, PH, (TAN ? F_TAN : 0) | (PF ? F_PREFETCH : 0), C::SW>;
+    auto k = k_sweep<V, PH, (TAN ? F_TAN : 0) | (PF ? F_PREFETCH : 0) | (NT ? F_NTLOAD : 0), C::SW>;
     hipLaunchKernelGGL(k, dim3(grid), dim3(1 << PH::plan.TT()), 0, st, psi, a);
     return hipGetLastError();
 }
 
-template <typename V, bool TAN, bool PF, int R = 3>
+template <typename V, bool TAN, bool PF, int R = 3, bool NT = false>
 hipError_t launch_high(V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
 {
     if constexpr (R > Cfg<V>::HTB - 1) {
         return hipErrorInvalidValue;
     } else {
-        if (rbits == R) return launch_high_r<V, R, TAN, PF>(psi, a, grid, st);
-        return launch_high<V, TAN, PF, R + 1>(psi, rbits, a, grid, st);
+        if (rbits == R) return launch_high_r<V, R, TAN, PF, NT>(psi, a, grid, st);
+        return launch_high<V, TAN, PF, R + 1, NT>(psi, rbits, a, grid, st);
     }
 }
 
+// nt: nontemporal amplitude loads (complex64 tan-form prefetching passes only)
 template <typename V>
-hipError_t launch_high_dyn(bool tan, bool pf, V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st)
+hipError_t launch_high_dyn(bool tan, bool pf, V* psi, int rbits, const SweepArgs& a, int grid, hipStream_t st,
+                           bool nt = false)
 {
+    if constexpr (std::is_same_v<V, float2>) {
+        if (nt) return tan && pf ? launch_high<V, true, true, 3, true>(psi, rbits, a, grid, st) : hipErrorInvalidValue;
+    }
+    if (nt) return hipErrorInvalidValue;
     if (tan) return pf ? launch_high<V, true, true>(psi, rbits, a, grid, st)
                        : launch_high<V, true, false>(psi, rbits, a, grid, st);
     return pf ? launch_high<V, false, true>(psi, rbits, a, grid, st)
@@ -1985,7 +1992,7 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
             const int g = clamp_grid(b.ntiles, tun.grid);
             if (tun.timer) tun.timer->begin(st);
             // the next tile's loads go out before this tile's rotations (N=30: 3.242 -> 3.225 ms)
-            e = launch_high_dyn<V>(tan, true, psi, A.R, b, g, st);
+            e = launch_high_dyn<V>(tan, true, psi, A.R, b, g, st, std::is_same_v<V, float2> && tun.nt_plain);
             if (e != hipSuccess) return e;
             if (tun.timer) tun.timer->end(st, "fused_high_a", pass_bytes);
         }
@@ -2396,6 +2403,10 @@ SweepTuning default_tuning(int n, int precision)
     // Complex128: 2^19 and 2^20 equal at N=26-28; N=30 15.24 -> 14.84 ms,
     // N=31 35.2 -> 31.6 ms, N=32 71.7 -> 67.7 ms.
     t.page_bits = n >= 31 ? 24 : 20;
+    // nontemporal loads in the complex64 plain high-group pass of a fused run:
+    // per step N=27 -1.8 %, N=28 -1.3 %, N=29 -1.1 %, N=30 unchanged, N=31 -1.1 %,
+    // N=32 -0.8 %; N=26 +0.4 % (same box)
+    t.nt_plain = precision == kC64 && n >= 27;
     return t;
 }
 

@@ -78,6 +78,9 @@ struct SweepTuning {
     // into LDS for n <= 29 (it fits beside the 64 KB exchange at two
     // workgroups per CU; default_tuning: n = 26..27)
     bool lds_tab13 = false;
+    // complex64 fused runs: the plain high-group pass loads nontemporally
+    // (default_tuning: n >= 27)
+    bool nt_plain = false;
     KernelTimer* timer = nullptr;
 };
 

@@ -320,6 +320,28 @@ def test_folded_finish_bit_identical(bk, n, prec):
         assert np.array_equal(out[0][key], out[1][key]), key
 
 
+@pytest.mark.parametrize("n", [24, 27])
+def test_nontemporal_plain_pass_bit_identical(bk, n):
+    """The complex64 plain high-group pass with nontemporal amplitude loads
+    (default from n = 27) against cached loads: only the cache policy differs,
+    so the state and every observable agree bit for bit."""
+    _, s0, e0 = bk.make_trap(n, 17)
+    out = {}
+    for nt in (1, 0):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("nt_plain", nt)
+            r = eng.evolve(n, s0, 0.35, 0.02, 8, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                           measure_every=1)
+            out[nt] = (r, eng.download())
+        finally:
+            eng.close()
+    (ra, pa), (rb, pb) = out[0], out[1]
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    for key in ("p_less", "dh", "norm"):
+        assert np.array_equal(ra[key], rb[key]), key
+
+
 def test_tuning_key_keeps_other_defaults(bk):
     """bk_set_tuning pins only the key it sets: a later state of another n or
     precision takes that state's defaults for every other key (row bits,
