@@ -306,6 +306,10 @@ __host__ __device__ constexpr int phase_tab_count(int n)
     return static_cast<int>(((static_cast<int64_t>(n - 1) * n * (2 * n - 1) / 6) >> 2) + 1);
 }
 
+// energy-cache words per load of a sweep thread holding NR codes (the cache
+// layout is described at pack8)
+__host__ __device__ constexpr int ec_vec_words(int NR) { return (3 * NR / 8) % 4 == 0 ? 4 : (3 * NR / 8) % 2 == 0 ? 2 : 1; }
+
 // complex64 passes over 2^14-amplitude tiles (the high-qubit groups and the
 // wide LOW tile) that apply the phase copy the phase table into LDS behind
 // the exchange buffer (at most 2605 entries, 20.8 KB at n = 32): the
@@ -530,30 +534,30 @@ __global__ __launch_bounds__(1 << PH::plan.TT()) void k_sweep(V* psi, const Swee
             }
         }
         if constexpr ((FLAGS & F_ECACHE) != 0) {
-            // ecb counts amplitudes: 3/8 of a word each (tile bases are multiples of 8)
+            // ecb counts amplitudes: 3/8 of a word each (tile bases are multiples
+            // of 8); vector q of this thread's words sits at q * VW << TT (store8)
+            constexpr int VW = ec_vec_words(NR);
             const uint32_t* ew = reinterpret_cast<const uint32_t*>(a.ecache) + (ecb / 8) * 3 +
-                                 static_cast<uint64_t>(t) * NEC;
-            if constexpr (NEC % 4 == 0) {
-                const uint4* ep = reinterpret_cast<const uint4*>(ew);
+                                 static_cast<uint64_t>(t) * VW;
+            if constexpr (VW == 4) {
 #pragma unroll
                 for (int q = 0; q < NEC / 4; ++q) {
-                    const uint4 w = ep[q];
+                    const uint4 w = *reinterpret_cast<const uint4*>(ew + (q << (TT + 2)));
                     ev[4 * q + 0] = w.x;
                     ev[4 * q + 1] = w.y;
                     ev[4 * q + 2] = w.z;
                     ev[4 * q + 3] = w.w;
                 }
-            } else if constexpr (NEC % 2 == 0) {
-                const uint2* ep = reinterpret_cast<const uint2*>(ew);
+            } else if constexpr (VW == 2) {
 #pragma unroll
                 for (int q = 0; q < NEC / 2; ++q) {
-                    const uint2 w = ep[q];
+                    const uint2 w = *reinterpret_cast<const uint2*>(ew + (q << (TT + 1)));
                     ev[2 * q + 0] = w.x;
                     ev[2 * q + 1] = w.y;
                 }
             } else {
 #pragma unroll
-                for (int q = 0; q < NEC; ++q) ev[q] = ew[q];
+                for (int q = 0; q < NEC; ++q) ev[q] = ew[q << TT];
             }
         }
     };
@@ -993,13 +997,33 @@ __global__ void k_set_basis(V* psi, uint64_t s0, double re, double im)
 // bits.  Eight codes share three 32-bit words (1.5 B per amplitude):
 //   w0 = m0 | m1 << 12 | m2 << 24, w1 = m2 >> 8 | m3 << 4 | m4 << 16 | m5 << 28,
 //   w2 = m5 >> 4 | m6 << 8 | m7 << 20.
-// A cache is laid out tile by tile, thread t's NR codes of its last-phase
-// registers contiguous (3 NR / 8 words), so a sweep reads them with wide loads.
+// A cache is laid out tile by tile.  Thread t of a sweep holds the NR codes
+// of its last-phase registers in 3 NR / 8 words, which it loads as vectors of
+// VW words; vector q of every thread of the tile precedes vector q + 1, so
+// each of a wave's cache loads reads one contiguous run of 64 VW words.
 __device__ __forceinline__ void pack8(uint32_t* __restrict__ w, const uint32_t (&m)[8])
 {
     w[0] = m[0] | (m[1] << 12) | (m[2] << 24);
     w[1] = (m[2] >> 8) | (m[3] << 4) | (m[4] << 16) | (m[5] << 28);
     w[2] = (m[5] >> 4) | (m[6] << 8) | (m[7] << 20);
+}
+
+// the three words of codes 8g .. 8g+7 (in the tile order of a TB/RB plan)
+template <int TB, int RB>
+__device__ __forceinline__ void store8(uint32_t* __restrict__ out, uint64_t g, const uint32_t (&m)[8])
+{
+    constexpr int NR = 1 << RB, TT = TB - RB, VW = ec_vec_words(NR);
+    uint32_t w[3];
+    pack8(w, m);
+    const uint64_t i0 = 8 * g;
+    const uint64_t tile_words = (i0 >> TB) * (3ull << (TB - 3));
+    const uint32_t rem = static_cast<uint32_t>(i0) & ((1u << TB) - 1u);
+    const uint32_t t = rem >> RB, k = (rem & (NR - 1u)) >> 3;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const uint32_t tw = 3 * k + q;  // word of thread t
+        out[tile_words + (tw / VW) * (static_cast<uint64_t>(VW) << TT) + t * VW + tw % VW] = w[q];
+    }
 }
 
 // LOW-sweep layout (plan PH)
@@ -1022,7 +1046,7 @@ __global__ __launch_bounds__(kThreads) void k_energy_fill(uint32_t* __restrict__
             for (int b = 0; b < RB; ++b) u |= ((j >> b) & 1u) << P.ph[NP - 1].reg[b];
             m[q] = labs_energy_u32(static_cast<uint32_t>((tile << TB) | u), n) >> 2;
         }
-        pack8(out + 3 * g, m);
+        store8<TB, RB>(out, g, m);
     }
 }
 
@@ -1057,7 +1081,7 @@ __global__ __launch_bounds__(kThreads) void k_energy_fill_high(uint32_t* __restr
             const uint64_t base = (lo << RR) | (mid << (h + a1)) | (hi << (h2 + (TB - RR - a1)));
             m[q] = labs_energy_u32(static_cast<uint32_t>(base + off), n) >> 2;
         }
-        pack8(out + 3 * g, m);
+        store8<TB, RB>(out, g, m);
     }
 }
 
