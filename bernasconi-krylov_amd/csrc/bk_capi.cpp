@@ -79,7 +79,11 @@ int apply_tuning_key(bk_ctx* c, bk::SweepTuning& t, const std::string& k, int va
     if (k == "grid") {
         if (value < 1 || value > (1 << 20)) return fail(BK_E_INVALID, "grid %d", value);
         t.grid = value;
-        HIPCALL(c->partials.ensure(bk::partials_bytes(value)));
+        HIPCALL(c->partials.ensure(bk::partials_bytes(bk::max_grid(t))));
+    } else if (k == "grid_bphase" || k == "grid_lphase") {
+        if (value < 0 || value > (1 << 20)) return fail(BK_E_INVALID, "%s %d", k.c_str(), value);
+        (k == "grid_bphase" ? t.grid_bphase : t.grid_lphase) = value;
+        HIPCALL(c->partials.ensure(bk::partials_bytes(bk::max_grid(t))));
     } else if (k == "row_bits") {
         if (value < 3 || value > 8) return fail(BK_E_INVALID, "row_bits %d", value);
         t.row_bits = value;
@@ -232,7 +236,7 @@ int ensure_state(bk_ctx* c, int n, int precision)
         if (rc) return rc;
     }
     c->tuning.timer = c->timing ? &c->timer : nullptr;
-    HIPCALL(c->partials.ensure(bk::partials_bytes(c->tuning.grid)));
+    HIPCALL(c->partials.ensure(bk::partials_bytes(bk::max_grid(c->tuning))));
     return BK_OK;
 }
 

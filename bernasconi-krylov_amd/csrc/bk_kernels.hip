@@ -1964,7 +1964,7 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
     const bool fold = rs.phase_tab2 != nullptr;
     // measured pass k writes partials region k & 1; the next pass of the run
     // finishes it into the step's slot (the last one, k_finish3)
-    const size_t region_doubles = 3 * static_cast<size_t>(std::max(tun.grid, 4096));
+    const size_t region_doubles = 3 * static_cast<size_t>(std::max(max_grid(tun), 4096));
     struct Fin {
         const double* part = nullptr;
         int count = 0;
@@ -2062,7 +2062,8 @@ hipError_t run_impl(V* psi, const RunSpec& rs, const SweepTuning& tun, double* p
                              group_tb(B) == Cfg<V>::HTB;
         const int fl = F_PHASE | F_ECACHE | F_TAN | (slot ? F_MEASURE : 0) | (post ? F_POSTROT : 0) |
                        (ldstab ? F_LDSTAB : 0) | (ldstabp ? F_LDSTABP : 0);
-        const int g = clamp_grid(b.ntiles, tun.grid);
+        const int gp = low ? tun.grid_lphase : tun.grid_bphase;
+        const int g = clamp_grid(b.ntiles, gp > 0 ? gp : tun.grid);
         if (tun.timer) tun.timer->begin(st);
         e = launch_ppass<V>(low, ltb, fl, group_tb(B), B.R, psi, b, g, st);
         if (e != hipSuccess) return e;
@@ -2433,12 +2434,22 @@ SweepTuning default_tuning(int n, int precision)
     // per step N=27 -1.8 %, N=28 -1.3 %, N=29 -1.1 %, N=30 unchanged, N=31 -1.1 %,
     // N=32 -0.8 %; N=26 +0.4 % (same box)
     t.nt_plain = precision == kC64 && n >= 27;
+    // complex64 phase passes from n = 30: the high group's (one workgroup per
+    // CU) as one persistent workgroup on each of the 256 CUs, the LOW tile's
+    // as 16384 workgroups (step N=30 6.957 -> 6.886 ms, N=31 14.28 -> 14.04 ms,
+    // N=32 unchanged; no gain at N=26-29, same box)
+    if (precision == kC64 && n >= 30) {
+        t.grid_bphase = 256;
+        t.grid_lphase = 16384;
+    }
     return t;
 }
 
 // two regions: a fused run's measured pass k writes region k & 1 while the
 // next pass finishes region (k - 1) & 1
 size_t partials_bytes(int grid) { return 2 * sizeof(double) * 3 * static_cast<size_t>(std::max(grid, 4096)); }
+
+int max_grid(const SweepTuning& t) { return std::max({t.grid, t.grid_bphase, t.grid_lphase}); }
 
 hipError_t launch_step(void* psi, void* psi_alt, int* result_in_alt, const StepSpec& spec,
                        const SweepTuning& tun, double* partials, hipStream_t stream)

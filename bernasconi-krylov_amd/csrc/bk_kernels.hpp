@@ -81,6 +81,10 @@ struct SweepTuning {
     // complex64 fused runs: the plain high-group pass loads nontemporally
     // (default_tuning: n >= 27)
     bool nt_plain = false;
+    // workgroups of a fused run's one-state phase passes over the high group
+    // and over the LOW tile (0: grid)
+    int grid_bphase = 0;
+    int grid_lphase = 0;
     KernelTimer* timer = nullptr;
 };
 
@@ -88,6 +92,8 @@ SweepTuning default_tuning(int n, int precision);
 
 // scratch for reductions (partials): bytes needed for a given grid
 size_t partials_bytes(int grid);
+// the largest workgroup count any pass of these settings launches with
+int max_grid(const SweepTuning& t);
 
 // Launch one Trotter step (phase + all rotations) on 'stream'.  With psi_alt
 // non-null the sweeps run out of place, alternating psi -> psi_alt -> psi ...;

@@ -452,6 +452,9 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  *              the state with nontemporal loads (default on from n = 27, where
  *              it shortens the step by up to 1.8 %); results are the same bit
  *              for bit.
+ * "grid_bphase" / "grid_lphase" : workgroups of a fused run's one-state phase
+ *              passes over the high group / the LOW tile (0 = "grid"; default
+ *              256 / 16384 for complex64 from n = 30, else 0).
  * "fold_finish" : 1 (default) = in a fused run, the pass after a measured
  *              phase pass sums that pass's partials in its first workgroup (and
  *              the reverse run's A pass the previous step's gradient partials);
