@@ -529,7 +529,8 @@ class Engine:
 
     def set_tuning(self, key: str, value: int):
         """Sweep knobs: 'grid', 'row_bits', 'page_bits', 'pingpong', 'prefetch_low',
-        'prefetch_high', 'fused', 'wide_low', 'narrow_phase', 'lds_tab13', 'nt_plain', 'grid_bphase', 'grid_lphase', 'fold_finish'
+        'prefetch_high', 'fused', 'wide_low', 'narrow_phase', 'lds_tab13', 'nt_plain', 'grid_bphase', 'grid_lphase', 'rev_grid_low',
+        'rev_grid_bphase', 'fold_finish'
         (see bk_set_tuning)."""
         _check(lib().bk_set_tuning(self._h, key.encode(), int(value)))
 

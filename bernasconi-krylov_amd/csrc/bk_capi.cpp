@@ -80,9 +80,10 @@ int apply_tuning_key(bk_ctx* c, bk::SweepTuning& t, const std::string& k, int va
         if (value < 1 || value > (1 << 20)) return fail(BK_E_INVALID, "grid %d", value);
         t.grid = value;
         HIPCALL(c->partials.ensure(bk::partials_bytes(bk::max_grid(t))));
-    } else if (k == "grid_bphase" || k == "grid_lphase") {
+    } else if (k == "grid_bphase" || k == "grid_lphase" || k == "rev_grid_low" || k == "rev_grid_bphase") {
         if (value < 0 || value > (1 << 20)) return fail(BK_E_INVALID, "%s %d", k.c_str(), value);
-        (k == "grid_bphase" ? t.grid_bphase : t.grid_lphase) = value;
+        (k == "grid_bphase" ? t.grid_bphase : k == "grid_lphase" ? t.grid_lphase
+                            : k == "rev_grid_low" ? t.rev_grid_low : t.rev_grid_bphase) = value;
         HIPCALL(c->partials.ensure(bk::partials_bytes(bk::max_grid(t))));
     } else if (k == "row_bits") {
         if (value < 3 || value > 8) return fail(BK_E_INVALID, "row_bits %d", value);

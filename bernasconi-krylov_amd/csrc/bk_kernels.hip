@@ -2123,6 +2123,15 @@ hipError_t launch_high_rev_ppass(V* phi, int htb, int rbits, const SweepArgs& a,
 // the last pass rotates by the identity (c = 1, s = 0, exact) after its phase.
 // Every qubit group of step i has written its partials into region i & 1 when
 // step i's phase pass ends, which is when the region is summed.
+// workgroups of a fused reverse run's passes over the LOW tile (tb bits),
+// group A and group B (the phase-carrying group), on 2^htb-amplitude tiles
+void reverse_grids(uint64_t dim, int tb, int htb, const SweepTuning& tun, int* gl, int* ga, int* gb)
+{
+    *gl = clamp_grid(dim >> tb, tun.rev_grid_low > 0 ? tun.rev_grid_low : tun.grid);
+    *ga = clamp_grid(dim >> htb, tun.grid);
+    *gb = clamp_grid(dim >> htb, tun.rev_grid_bphase > 0 ? tun.rev_grid_bphase : tun.grid);
+}
+
 template <typename V>
 hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTuning& tun, double* gpart,
                             hipStream_t st)
@@ -2147,7 +2156,7 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
     }
     const bool two = grp.size() == 2;
     const HighSweep A = two ? grp[0] : HighSweep{};
-    const HighSweep B = grp.back();
+    const HighSweep& B = grp.back();
     // complex64 runs use the tan butterfly when every step allows it: 4 FMAs per
     // pair instead of 8 operations, with the step's c^n and the rounding
     // feedback in the phase table and c^2m weights on the gradient terms
@@ -2157,8 +2166,11 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
     a.tab = tan ? rs.fb_applied : rs.phase_tab;
     a.n = n;
     a.psi2 = chi;
-    const int gL = clamp_grid(dim >> C::TB, tun.grid), gH = clamp_grid(dim >> C::HTB, tun.grid);
-    const int offA = gL, offB = gL + (two ? gH : 0), GS = offB + gH;
+    // every pass over one tile kind (LOW, group A, group B) uses one workgroup
+    // count, so each kind's partials slot of a region has a fixed size
+    int gL = 0, gA = 0, gB = 0;
+    reverse_grids(dim, C::TB, C::HTB, tun, &gL, &gA, &gB);
+    const int offA = gL, offB = gL + (two ? gA : 0), GS = offB + gB;
     auto region = [&](int i) { return gpart + static_cast<size_t>(i & 1) * static_cast<size_t>(GS); };
     // coefficients of step i: (c, s), or (unused, t) with c^2 = 1 / (1 + t_f^2)
     // for the FP32 tangent the kernel uses
@@ -2206,6 +2218,7 @@ hipError_t reverse_run_impl(V* phi, V* chi, const RevRunSpec& rs, const SweepTun
     // group included); plain passes on 2^HTB-amplitude tiles
     auto high_rev = [&](bool phase, const HighSweep& hs, const SweepArgs& b) -> hipError_t {
         const int R = hs.R, htb = group_tb(hs);
+        const int gH = &hs == &B ? gB : gA;
         if (!phase && htb != C::HTB) return hipErrorInvalidValue;
         if constexpr (std::is_same_v<V, float2>) {
             if (tan && phase && ldstab && htb == kNarrowTB)
@@ -2442,6 +2455,13 @@ SweepTuning default_tuning(int n, int precision)
         t.grid_bphase = 256;
         t.grid_lphase = 16384;
     }
+    // complex64 reverse runs at n = 26: 512 workgroups for the LOW tile's
+    // passes, 16384 for the phase-carrying group's (backward 0.550 -> 0.542 s
+    // over 640 steps, same box); slower at n = 24, 25, 27 and 28
+    if (precision == kC64 && n == 26) {
+        t.rev_grid_low = 512;
+        t.rev_grid_bphase = 16384;
+    }
     return t;
 }
 
@@ -2474,8 +2494,9 @@ size_t reverse_partials_count(int n, int precision, const SweepTuning& tun)
     for (size_t i = 0; i < plan_high_sweeps(n, tb, htb, std::max(tun.row_bits, 3), tun.page_bits).size(); ++i)
         total += static_cast<size_t>(clamp_grid(dim >> htb, tun.grid));
     // fused reverse runs: two regions of LOW + two high groups
-    const size_t fused = 2 * (static_cast<size_t>(clamp_grid(dim >> tb, tun.grid)) +
-                              2 * static_cast<size_t>(clamp_grid(dim >> htb, tun.grid)));
+    int gl = 0, ga = 0, gb = 0;
+    reverse_grids(dim, tb, htb, tun, &gl, &ga, &gb);
+    const size_t fused = 2 * (static_cast<size_t>(gl) + static_cast<size_t>(ga) + static_cast<size_t>(gb));
     return std::max(total, fused);
 }
 

@@ -85,6 +85,10 @@ struct SweepTuning {
     // and over the LOW tile (0: grid)
     int grid_bphase = 0;
     int grid_lphase = 0;
+    // workgroups of a fused reverse run's passes over the LOW tile and over
+    // the phase-carrying group (0: grid)
+    int rev_grid_low = 0;
+    int rev_grid_bphase = 0;
     KernelTimer* timer = nullptr;
 };
 
