@@ -2456,12 +2456,9 @@ SweepTuning default_tuning(int n, int precision)
         t.grid_lphase = 16384;
     }
     // complex64 reverse runs at n = 26: 512 workgroups for the LOW tile's
-    // passes, 16384 for the phase-carrying group's (backward 0.550 -> 0.542 s
-    // over 640 steps, same box); slower at n = 24, 25, 27 and 28
-    if (precision == kC64 && n == 26) {
-        t.rev_grid_low = 512;
-        t.rev_grid_bphase = 16384;
-    }
+    // passes (backward 0.550 -> 0.542 s over 640 steps, same box); slower at
+    // n = 24, 25, 27 and 28
+    if (precision == kC64 && n == 26) t.rev_grid_low = 512;
     return t;
 }
 

@@ -456,8 +456,9 @@ int bk_set_energy_mode(bk_ctx* ctx, int mode);
  *              passes over the high group / the LOW tile (0 = "grid"; default
  *              256 / 16384 for complex64 from n = 30, else 0).
  * "rev_grid_low" / "rev_grid_bphase" : workgroups of a fused reverse run's
- *              passes over the LOW tile / the phase-carrying group (0 = "grid";
- *              default 512 / 16384 for complex64 at n = 26, else 0).
+ *              passes over the LOW tile / the phase-carrying group (0 = "grid",
+ *              at most one workgroup per 2^14 amplitudes for the group;
+ *              rev_grid_low defaults to 512 for complex64 at n = 26, else 0).
  * "fold_finish" : 1 (default) = in a fused run, the pass after a measured
  *              phase pass sums that pass's partials in its first workgroup (and
  *              the reverse run's A pass the previous step's gradient partials);
