@@ -242,6 +242,30 @@ def test_narrow_phase_group_reverse(bk):
     assert np.abs(g1 - g0).max() <= 1e-6 * np.abs(g0).max()
 
 
+def test_reverse_low_grid(bk):
+    """The reverse run's LOW-tile passes on 512 workgroups (rev_grid_low, the
+    n = 26 default) against the default grid at n = 24: every tile is
+    transformed the same way, so the drift is the same; the gradient partials
+    of the LOW passes sum in another grouping (per-workgroup FP64 partials),
+    so the gradients agree to rounding."""
+    n = 24
+    _, s0, e0 = bk.make_trap(n, 5)
+    gs = np.repeat(_fields(4, 93), 10)
+    out = {}
+    for g in (0, 512):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("rev_grid_low", g)
+            eng.adjoint_forward(n, s0, gs, 0.02, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                                checkpoint_every=16)
+            out[g] = eng.adjoint_backward(1.0, 0.25)
+        finally:
+            eng.close()
+    (g0, d0), (g1, d1) = out[0], out[512]
+    assert d0 == d1
+    assert np.abs(g1 - g0).max() <= 1e-10 * np.abs(g0).max()
+
+
 def test_backward_leaves_state_and_detects_stale_tape(bk, engine):
     n = 13
     _, s0, e0 = bk.make_trap(n, 1)

@@ -342,6 +342,31 @@ def test_nontemporal_plain_pass_bit_identical(bk, n):
         assert np.array_equal(ra[key], rb[key]), key
 
 
+def test_phase_pass_grids(bk):
+    """Per-pass workgroup counts of the phase passes (grid_bphase /
+    grid_lphase, the defaults from n = 30) set at n = 24, beyond the default
+    grid for the LOW pass: every tile is transformed the same way, so the state
+    is bit-identical; only the order in which the workgroups' FP64 partials
+    are summed changes, so the observables agree to rounding."""
+    n = 24
+    _, s0, e0 = bk.make_trap(n, 19)
+    out = {}
+    for grids in ((0, 0), (256, 16384)):
+        eng = bk.Engine(0)
+        try:
+            eng.set_tuning("grid_bphase", grids[0])
+            eng.set_tuning("grid_lphase", grids[1])
+            r = eng.evolve(n, s0, 0.35, 0.02, 12, order=bk.BK_STRANG, precision=bk.BK_C64, e0=e0,
+                           measure_every=1)
+            out[grids] = (r, eng.download())
+        finally:
+            eng.close()
+    (ra, pa), (rb, pb) = out[(0, 0)], out[(256, 16384)]
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    for key in ("p_less", "dh", "norm"):
+        np.testing.assert_allclose(ra[key], rb[key], rtol=1e-12, atol=1e-15, err_msg=key)
+
+
 def test_tuning_key_keeps_other_defaults(bk):
     """bk_set_tuning pins only the key it sets: a later state of another n or
     precision takes that state's defaults for every other key (row bits,
